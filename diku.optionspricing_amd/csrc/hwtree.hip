@@ -1,0 +1,682 @@
+// hwtree.hip -- kernels and C ABI of the MI355X-native HW1F trinomial pricer.
+//
+// Pipeline of one execution (captured once into a HIP graph per plan):
+//   k_reset    clear queue head, histogram, tallies (a kernel, not a memset
+//              node, so graph replays are safe)
+//   k_setup    one thread per option: tree constants (the reference's
+//              OptionConstants), P(e dt) and P(n dt) from the yield curve, edge
+//              factors, work estimate; trivial options are priced right here
+//   k_scan     bucket offsets, largest work first (longest-processing-time order)
+//   k_scatter  option indices into the task list
+//   k_price_wave  persistent kernel: each wavefront pulls options off the task
+//              list and prices them with the on-chip tree column (hwt_wave.h)
+//
+// Pricing identity.  With Q_e the Arrow-Debreu prices at the exercise step e
+// and B_e the bond values there, the reference's backward induction gives
+// price = sum_j Q_e[j] payoff(B_e[j]) exactly (Arrow-Debreu duality, same tree,
+// same discount factors).  Because alpha only shifts every node of a step by
+// the same discount, Q_e = (P(e dt)/S1) y_e and B_e = 100 S1 P(n dt)/(S2 P(e dt)) b_e
+// where y_e is the undiscounted forward state after e steps, b_e the
+// undiscounted backward state after n-e steps, S1 = sum y_e and S2 = y_e . b_e.
+// So each option costs n stencil steps (e forward + n-e backward) with no
+// per-step reductions, against 2n steps plus n reductions in seq/Seq.hpp.
+#include "hwt_wave.h"
+#include "../../include/hwtree.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace hwt;
+
+namespace {
+
+struct DevOpts {
+    int32_t n;
+    const double *strike, *maturity, *length;
+    const uint16_t *term_unit, *term_steps;
+    const double *a, *sigma;
+    const int8_t *type;
+};
+
+struct DevYield {
+    int32_t n;
+    const double *rate;
+    const int32_t *day;
+};
+
+constexpr int NBUCKET = 256;
+enum Tally { T_QUEUE = 0, T_NTASK = 1, T_INVALID = 2, T_UNSUP = 3, T_WORK = 4, T_SEQWORK = 6, T_NCOUNT = 8 };
+
+__device__ __forceinline__ int bucket_of(int64_t key)
+{
+    const int b = (int)(8.0 * log2((double)key + 1.0));
+    return b < 0 ? 0 : (b >= NBUCKET ? NBUCKET - 1 : b);
+}
+
+__global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i < T_NCOUNT) tally[i] = 0;
+    if (i < NBUCKET) {
+        hist[i] = 0;
+        cursor[i] = 0;
+    }
+}
+
+__global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, OptConst *__restrict__ cst,
+                        int32_t *__restrict__ bucket, int32_t *__restrict__ hist, int32_t *__restrict__ tally,
+                        double *__restrict__ out)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= o.n) return;
+    OptConst c;
+    const int32_t st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i],
+                                        o.term_steps[i], o.a[i], o.sigma[i], o.type[i], maxWidth, c);
+    c.status = st;
+    int b = -1;
+    if (st == ST_INVALID) {
+        out[i] = NAN;
+        atomicAdd(&tally[T_INVALID], 1);
+    } else if (st == ST_UNSUPPORTED) {
+        out[i] = NAN;
+        atomicAdd(&tally[T_UNSUP], 1);
+    } else if (c.n == 0) {
+        out[i] = 100.0;  // no steps: the face value (seq/Seq.hpp:119,171)
+    } else {
+        const int tu = o.term_unit[i];
+        const double tn = c.n * c.dt;
+        c.Pn = exp(-yield_at(tn, tu, y.rate, y.day, y.n) * tn);
+        if (c.e < 0) {
+            out[i] = 100.0 * c.Pn;  // exercise outside the tree: the zero-coupon bond
+        } else {
+            const double te = c.e * c.dt;
+            c.Pe = exp(-yield_at(te, tu, y.rate, y.day, y.n) * te);
+            TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
+            edge_factors(t);
+            c.rho = t.rho;
+            c.rhop = t.rhop;
+            c.work = dual_node_updates(c.n, c.e, c.jmax);
+            const int64_t seqw = seq_node_updates(c.n, c.jmax);
+            atomicAdd((unsigned long long *)&tally[T_WORK], (unsigned long long)c.work);
+            atomicAdd((unsigned long long *)&tally[T_SEQWORK], (unsigned long long)seqw);
+            int64_t key;
+            switch (sort) {
+            case 'W': case 'w': key = c.w; break;
+            case 'H': case 'h': key = c.n; break;
+            case '-': key = 0; break;
+            default: key = c.work; break;
+            }
+            b = bucket_of(key);
+            atomicAdd(&hist[b], 1);
+        }
+    }
+    cst[i] = c;
+    bucket[i] = b;
+}
+
+// Exclusive offsets of the buckets, visited largest key first unless ascending.
+__global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ offs, int32_t *__restrict__ tally, int32_t asc)
+{
+    __shared__ int32_t s[NBUCKET];
+    const int tid = threadIdx.x;
+    const int idx = asc ? tid : NBUCKET - 1 - tid;
+    const int32_t v = hist[idx];
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < NBUCKET; off <<= 1) {
+        const int32_t add = tid >= off ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += add;
+        __syncthreads();
+    }
+    offs[idx] = s[tid] - v;
+    if (tid == NBUCKET - 1) tally[T_NTASK] = s[tid];
+}
+
+__global__ void k_scatter(int32_t n, const int32_t *__restrict__ bucket, const int32_t *__restrict__ offs,
+                          int32_t *__restrict__ cursor, int32_t *__restrict__ order)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= n) return;
+    const int b = bucket[i];
+    if (b >= 0) order[offs[b] + atomicAdd(&cursor[b], 1)] = i;
+}
+
+__device__ double price_option(const OptConst &c, int lane, lds_f64 *ldsA, lds_f64 *ldsB, lds_f64 *etab)
+{
+    const TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
+    fill_edge_table(t, lane, etab);
+    int flo, fhi, blo, bhi;
+    forward_pass(t, c.e, lane, ldsA, etab, flo, fhi);
+    backward_pass(t, c.n, c.e, lane, ldsB, etab, blo, bhi);
+
+    const int ep = min(c.e, c.jmax);
+    const int klo = c.jmax - ep, khi = c.jmax + ep;
+    auto fy = [&](int k) { return (k >= flo && k < fhi) ? ldsA[k] : 0.0; };
+    auto fb = [&](int k) { return (k >= blo && k < bhi) ? ldsB[k] : 0.0; };
+    auto yk = [&](int k) {
+        double v = fy(k);
+        if (k == 1) v -= t.rho * fy(0);
+        if (k == t.w - 2) v -= t.rhop * fy(t.w - 1);
+        return v;
+    };
+    auto bk = [&](int k) {
+        double v = fb(k);
+        if (k == 0) v += t.rho * fb(1);
+        if (k == t.w - 1) v += t.rhop * fb(t.w - 2);
+        return v;
+    };
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = klo + lane; k <= khi; k += WAVE) {
+        const double yv = yk(k);
+        s1 += yv;
+        s2 += yv * bk(k);
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    const double scale = 100.0 * s1 * c.Pn / (s2 * c.Pe);
+    double s3 = 0.0;
+    for (int k = klo + lane; k <= khi; k += WAVE) {
+        const double B = scale * bk(k);
+        double v = (c.type == 1) ? c.X - B : B - c.X;  // PUT : CALL
+        v = (v < 0.0) ? 0.0 : v;
+        if (v == INFINITY) v = DBL_MAX;  // common/Domain.hpp:163-168
+        s3 += yk(k) * v;
+    }
+    s3 = wave_sum(s3);
+    return c.Pe / s1 * s3;
+}
+
+__global__ __launch_bounds__(WAVE) void k_price_wave(const OptConst *__restrict__ cst, const int32_t *__restrict__ order,
+                                                     int32_t *__restrict__ tally, double *__restrict__ out)
+{
+    __shared__ double ldsA[WAVE_MAX_WIDTH];
+    __shared__ double ldsB[WAVE_MAX_WIDTH];
+    __shared__ double etab[EDGE_TAB];
+    const int lane = threadIdx.x;
+    const int ntask = __builtin_amdgcn_readfirstlane(tally[T_NTASK]);
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&tally[T_QUEUE], 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, WAVE));
+        if (t >= ntask) break;
+        const int o = __builtin_amdgcn_readfirstlane(order[t]);
+        const OptConst c = cst[o];
+        const double p = price_option(c, lane, (lds_f64 *)ldsA, (lds_f64 *)ldsB, (lds_f64 *)etab);
+        if (lane == 0) out[o] = p;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- alphas
+// Per-step calibrated alphas of one option (the reference's forward pass,
+// seq/Seq.hpp:56-106): alpha_i dt = A_i - A_{i-1}, A_i = ln(sum_k E_k y_i[k]) + R(t_{i+1}) t_{i+1}.
+template <int K>
+__device__ __noinline__ void alphas_full(const OptConst c, const DevYield y, int tu, int lane, double *dst,
+                                         lds_f64 *buf, const lds_f64 *etab)
+{
+    const TreeShape t = uni(TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop});
+    double z[K], cl[K], cm[K], cr[K], wE[K];
+    window_coefs<K, true>(t, 0, lane, buf, etab, cl, cm, cr);
+    init_window<K, true>(z, t, 0, lane);
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const int k = lane * K + s;
+        double v = (k < c.w) ? node_disc(k, c.jmax, c.dr, c.dt) : 0.0;
+        if (k == 0) v -= c.rho * node_disc(1, c.jmax, c.dr, c.dt);
+        if (k == c.w - 1) v -= c.rhop * node_disc(c.w - 2, c.jmax, c.dr, c.dt);
+        wE[s] = v;
+    }
+    double Aprev = 0.0;
+    for (int i = 0; i <= c.n; ++i) {
+        if (i > 0) stencil_steps<K>(z, cl, cm, cr, 1);
+        double part = 0.0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) part += wE[s] * z[s];
+        const double sigma = wave_sum(part);
+        const double ti = (i + 1) * c.dt;
+        const double A = log(sigma) + yield_at(ti, tu, y.rate, y.day, y.n) * ti;
+        if (lane == 0) dst[i] = (A - Aprev) / c.dt;
+        Aprev = A;
+    }
+}
+
+__global__ __launch_bounds__(WAVE) void k_alphas(DevOpts o, DevYield y, const int64_t *__restrict__ offsets,
+                                                 int32_t *__restrict__ tally, double *__restrict__ alphas)
+{
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x;
+    OptConst c;
+    const int st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i], o.term_steps[i],
+                                    o.a[i], o.sigma[i], o.type[i], WAVE_MAX_WIDTH, c);
+    if (st != ST_OK) {
+        if (lane == 0) atomicAdd(&tally[st == ST_INVALID ? T_INVALID : T_UNSUP], 1);
+        return;
+    }
+    TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
+    edge_factors(t);
+    c.rho = t.rho;
+    c.rhop = t.rhop;
+    __shared__ double etab[EDGE_TAB];
+    __shared__ double buf[WAVE_MAX_WIDTH];
+    fill_edge_table(t, lane, (lds_f64 *)etab);
+    double *dst = alphas + offsets[i];
+    const int K = (c.w + WAVE - 1) / WAVE;
+    switch (K) {
+#define HWT_C(KK) case KK: alphas_full<KK>(c, y, o.term_unit[i], lane, dst, (lds_f64 *)buf, (const lds_f64 *)etab); break;
+        HWT_KCASES(HWT_C)
+#undef HWT_C
+    default: break;
+    }
+}
+
+// ------------------------------------------------------------- host side
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HWT_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t err_ = (call);                                                              \
+        if (err_ != hipSuccess)                                                                \
+            return fail(HWT_EHIP, std::string(#call) + ": " + hipGetErrorString(err_));        \
+    } while (0)
+
+bool valid_opts_ptrs(const hwt_options_f64 *o)
+{
+    return o && o->n >= 0 &&
+           (o->n == 0 || (o->strike && o->maturity && o->length && o->term_unit && o->term_steps && o->a &&
+                          o->sigma && o->type));
+}
+
+bool valid_yield_ptrs(const hwt_yield_f64 *y) { return y && y->n > 0 && y->rate && y->day; }
+
+DevOpts dev_opts(const hwt_options_f64 &o)
+{
+    return DevOpts{o.n, o.strike, o.maturity, o.length, o.term_unit, o.term_steps, o.a, o.sigma, o.type};
+}
+
+} // namespace
+
+struct hwt_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t capacity = 0, ycap = 0;
+    hwt_config cfg{};
+    int grid_wave = 0;
+    // device workspace, one allocation
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    OptConst *cst = nullptr;
+    int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *offs = nullptr, *tally = nullptr;
+    int32_t *h_tally = nullptr;  // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+    // captured graph and the arguments it was captured with
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    DevOpts g_opts{};
+    DevYield g_yield{};
+    double *g_out = nullptr;
+    hipStream_t g_stream = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool timed_kernel = false;
+};
+
+static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, double *out, hipStream_t s, bool time_kernel)
+{
+    const int n = o.n;
+    const int sort = p->cfg.sort ? p->cfg.sort : 'L';
+    const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
+    k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
+    if (n > 0) {
+        const int blocks = (n + 255) / 256;
+        k_setup<<<blocks, 256, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->bucket, p->hist, p->tally, out);
+        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
+        k_scatter<<<blocks, 256, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
+        k_price_wave<<<p->grid_wave, WAVE, 0, s>>>(p->cst, p->order, p->tally, out);
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
+    }
+    HWT_HIP(hipGetLastError());
+    return HWT_OK;
+}
+
+extern "C" {
+
+void hwt_config_init(hwt_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->form = HWT_FORM_AUTO;
+    cfg->precision = HWT_PREC_FP64;
+    cfg->sort = HWT_SORT_WORK_DESC;
+    cfg->device = -1;
+}
+
+int32_t hwt_abi_version(void) { return HWT_ABI_VERSION; }
+
+int32_t hwt_max_width(int32_t form)
+{
+    (void)form;
+    return WAVE_MAX_WIDTH;
+}
+
+const char *hwt_strerror(int code)
+{
+    switch (code) {
+    case HWT_OK: return "success";
+    case HWT_EINVAL: return "invalid argument";
+    case HWT_EUNSUPPORTED: return "unsupported option for the selected kernel form";
+    case HWT_ENOMEM: return "out of memory or capacity exceeded";
+    case HWT_EHIP: return "HIP runtime error";
+    default: return "unknown error";
+    }
+}
+
+const char *hwt_last_error(void) { return g_last_error.c_str(); }
+
+int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, const hwt_config *cfg)
+{
+    g_last_error.clear();
+    if (!out || capacity < 0 || yield_capacity < 1) return fail(HWT_EINVAL, "hwt_plan_create: bad arguments");
+    *out = nullptr;
+    hwt_plan *p = new (std::nothrow) hwt_plan();
+    if (!p) return fail(HWT_ENOMEM, "hwt_plan_create: host allocation");
+    if (cfg) p->cfg = *cfg;
+    else hwt_config_init(&p->cfg);
+    if (p->cfg.form < HWT_FORM_AUTO || p->cfg.form > HWT_FORM_WAVE || p->cfg.precision != HWT_PREC_FP64) {
+        delete p;
+        return fail(HWT_EUNSUPPORTED, "hwt_plan_create: form/precision not available");
+    }
+    p->capacity = capacity;
+    p->ycap = yield_capacity;
+    int rc = HWT_OK;
+    auto bail = [&](int code, const std::string &msg) {
+        hwt_plan_destroy(p);
+        return fail(code, msg);
+    };
+    if (hipGetDevice(&p->device) != hipSuccess) return bail(HWT_EHIP, "hipGetDevice failed");
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(HWT_EHIP, "hipStreamCreate failed");
+    const size_t cap = (size_t)std::max(capacity, 1);
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_cst = carve(cap * sizeof(OptConst));
+    const size_t o_bucket = carve(cap * sizeof(int32_t));
+    const size_t o_order = carve(cap * sizeof(int32_t));
+    const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
+    const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
+    const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
+    const size_t o_tally = carve(T_NCOUNT * sizeof(int32_t));
+    p->ws_bytes = off;
+    if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) return bail(HWT_ENOMEM, "hipMalloc workspace failed");
+    char *base = (char *)p->ws;
+    p->cst = (OptConst *)(base + o_cst);
+    p->bucket = (int32_t *)(base + o_bucket);
+    p->order = (int32_t *)(base + o_order);
+    p->hist = (int32_t *)(base + o_hist);
+    p->cursor = (int32_t *)(base + o_cursor);
+    p->offs = (int32_t *)(base + o_offs);
+    p->tally = (int32_t *)(base + o_tally);
+    if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        return bail(HWT_ENOMEM, "hipHostMalloc failed");
+    if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
+        hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess)
+        return bail(HWT_EHIP, "hipEventCreate failed");
+    // Persistent grid: as many single-wave workgroups as can be resident.
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_price_wave, WAVE, 0) != hipSuccess || per_cu < 1)
+        per_cu = 4;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus < 1)
+        cus = 256;
+    p->grid_wave = per_cu * cus;
+    *out = p;
+    (void)rc;
+    return HWT_OK;
+}
+
+void hwt_plan_destroy(hwt_plan *p)
+{
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->evk0) (void)hipEventDestroy(p->evk0);
+    if (p->evk1) (void)hipEventDestroy(p->evk1);
+    if (p->h_tally) (void)hipHostFree(p->h_tally);
+    if (p->ws) (void)hipFree(p->ws);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_f64 *dyield, double *dout, void *stream)
+{
+    g_last_error.clear();
+    if (!p || !valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
+        return fail(HWT_EINVAL, "hwt_plan_execute: null or negative-size argument");
+    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "hwt_plan_execute: more options than the plan capacity");
+    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "hwt_plan_execute: yield curve longer than the plan capacity");
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    const DevOpts o = dev_opts(*dopts);
+    const DevYield y{dyield->n, dyield->rate, dyield->day};
+    p->last_stream = s;
+    HWT_HIP(hipEventRecord(p->ev0, s));
+    if (p->cfg.flags & HWT_FLAG_NO_GRAPH) {
+        const int rc = launch_pipeline(p, o, y, dout, s, true);
+        if (rc) return rc;
+        p->timed_kernel = o.n > 0;
+    } else {
+        const bool same = p->exec && p->g_stream == s && p->g_out == dout &&
+                          std::memcmp(&p->g_opts, &o, sizeof(o)) == 0 && std::memcmp(&p->g_yield, &y, sizeof(y)) == 0;
+        if (!same) {
+            if (p->exec) (void)hipGraphExecDestroy(p->exec);
+            if (p->graph) (void)hipGraphDestroy(p->graph);
+            p->exec = nullptr;
+            p->graph = nullptr;
+            HWT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_pipeline(p, o, y, dout, s, false);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return fail(HWT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+            p->graph = g;
+            HWT_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+            p->g_opts = o;
+            p->g_yield = y;
+            p->g_out = dout;
+            p->g_stream = s;
+        }
+        HWT_HIP(hipGraphLaunch(p->exec, s));
+        p->timed_kernel = false;
+    }
+    HWT_HIP(hipEventRecord(p->ev1, s));
+    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    return HWT_OK;
+}
+
+int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
+{
+    if (!p) return fail(HWT_EINVAL, "hwt_plan_sync: null plan");
+    hipStream_t s = p->last_stream ? p->last_stream : p->stream;
+    HWT_HIP(hipStreamSynchronize(s));
+    const int32_t *t = p->h_tally;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p->ev0, p->ev1) == hipSuccess) stats->total_ms = ms;
+        stats->kernel_ms = stats->total_ms;
+        if (p->timed_kernel && hipEventElapsedTime(&ms, p->evk0, p->evk1) == hipSuccess) stats->kernel_ms = ms;
+        stats->device_memory = (int64_t)p->ws_bytes;
+        int64_t w, sw;
+        std::memcpy(&w, t + T_WORK, sizeof(w));
+        std::memcpy(&sw, t + T_SEQWORK, sizeof(sw));
+        stats->node_updates = w;
+        stats->seq_node_updates = sw;
+        stats->tasks = t[T_NTASK];
+        stats->unsupported = t[T_UNSUP];
+    }
+    if (t[T_INVALID] > 0) return fail(HWT_EINVAL, std::to_string(t[T_INVALID]) + " option(s) with invalid parameters (price NaN)");
+    if (t[T_UNSUP] > 0)
+        return fail(HWT_EUNSUPPORTED, std::to_string(t[T_UNSUP]) + " option(s) wider than " +
+                                          std::to_string(WAVE_MAX_WIDTH) + " nodes (price NaN)");
+    return HWT_OK;
+}
+
+// Device copies of host SoA arrays, one allocation.
+struct DeviceBatch {
+    void *mem = nullptr;
+    hwt_options_f64 o{};
+    hwt_yield_f64 y{};
+    double *out = nullptr;
+    ~DeviceBatch()
+    {
+        if (mem) (void)hipFree(mem);
+    }
+};
+
+static int upload(const hwt_options_f64 *h, const hwt_yield_f64 *hy, DeviceBatch &d, bool need_out)
+{
+    const size_t n = (size_t)h->n, ny = (size_t)hy->n;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_strike = carve(n * 8), o_mat = carve(n * 8), o_len = carve(n * 8), o_tu = carve(n * 2),
+                 o_ts = carve(n * 2), o_a = carve(n * 8), o_sig = carve(n * 8), o_type = carve(n),
+                 o_rate = carve(ny * 8), o_day = carve(ny * 4), o_out = carve(need_out ? n * 8 : 0);
+    if (hipMalloc(&d.mem, std::max<size_t>(off, 256)) != hipSuccess) return fail(HWT_ENOMEM, "hipMalloc batch failed");
+    char *b = (char *)d.mem;
+    auto put = [&](size_t at, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(b + at, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    if (put(o_strike, h->strike, n * 8) || put(o_mat, h->maturity, n * 8) || put(o_len, h->length, n * 8) ||
+        put(o_tu, h->term_unit, n * 2) || put(o_ts, h->term_steps, n * 2) || put(o_a, h->a, n * 8) ||
+        put(o_sig, h->sigma, n * 8) || put(o_type, h->type, n) || put(o_rate, hy->rate, ny * 8) ||
+        put(o_day, hy->day, ny * 4))
+        return fail(HWT_EHIP, "hipMemcpy H2D failed");
+    d.o = hwt_options_f64{h->n, (double *)(b + o_strike), (double *)(b + o_mat), (double *)(b + o_len),
+                          (uint16_t *)(b + o_tu), (uint16_t *)(b + o_ts), (double *)(b + o_a),
+                          (double *)(b + o_sig), (int8_t *)(b + o_type)};
+    d.y = hwt_yield_f64{hy->n, (double *)(b + o_rate), (int32_t *)(b + o_day)};
+    d.out = (double *)(b + o_out);
+    return HWT_OK;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    bool set = false;
+    int enter(int dev)
+    {
+        if (dev < 0) return HWT_OK;
+        if (hipGetDevice(&prev) != hipSuccess) return fail(HWT_EHIP, "hipGetDevice failed");
+        if (hipSetDevice(dev) != hipSuccess) return fail(HWT_EINVAL, "hipSetDevice: bad device ordinal");
+        set = true;
+        return HWT_OK;
+    }
+    ~DeviceGuard()
+    {
+        if (set) (void)hipSetDevice(prev);
+    }
+};
+
+int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, double *out_prices, const hwt_config *cfg,
+                  hwt_stats *stats)
+{
+    g_last_error.clear();
+    if (!valid_opts_ptrs(opts) || !valid_yield_ptrs(yield) || (opts->n > 0 && !out_prices))
+        return fail(HWT_EINVAL, "hwt_price_f64: null or negative-size argument");
+    hwt_config c;
+    if (cfg) c = *cfg;
+    else hwt_config_init(&c);
+    DeviceGuard g;
+    int rc = g.enter(c.device);
+    if (rc) return rc;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (opts->n == 0) return HWT_OK;
+    DeviceBatch d;
+    rc = upload(opts, yield, d, true);
+    if (rc) return rc;
+    c.flags |= HWT_FLAG_NO_GRAPH;  // one execution: a graph would not be replayed
+    hwt_plan *p = nullptr;
+    rc = hwt_plan_create(&p, opts->n, yield->n, &c);
+    if (rc) return rc;
+    rc = hwt_plan_execute(p, &d.o, &d.y, d.out, nullptr);
+    int rs = rc ? rc : hwt_plan_sync(p, stats);
+    const std::string keep = g_last_error;
+    if (rc == HWT_OK) {
+        if (hipMemcpy(out_prices, d.out, (size_t)opts->n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rs = fail(HWT_EHIP, "hipMemcpy D2H failed");
+        else
+            g_last_error = keep;
+    }
+    hwt_plan_destroy(p);
+    return rs;
+}
+
+int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, int64_t *offsets, double *alphas)
+{
+    g_last_error.clear();
+    if (!valid_opts_ptrs(opts) || !valid_yield_ptrs(yield) || !offsets)
+        return fail(HWT_EINVAL, "hwt_alphas_f64: null or negative-size argument");
+    // Heights on the host, with the same constant formulas as the device.
+    offsets[0] = 0;
+    for (int i = 0; i < opts->n; ++i) {
+        OptConst c;
+        const int st = option_constants(opts->strike[i], opts->maturity[i], opts->length[i], opts->term_unit[i],
+                                        opts->term_steps[i], opts->a[i], opts->sigma[i], opts->type[i],
+                                        WAVE_MAX_WIDTH, c);
+        if (st == ST_INVALID) return fail(HWT_EINVAL, "hwt_alphas_f64: option " + std::to_string(i) + " invalid");
+        if (st == ST_UNSUPPORTED)
+            return fail(HWT_EUNSUPPORTED, "hwt_alphas_f64: option " + std::to_string(i) + " too wide");
+        offsets[i + 1] = offsets[i] + (int64_t)c.n + 1;
+    }
+    if (!alphas || opts->n == 0) return HWT_OK;
+    DeviceBatch d;
+    int rc = upload(opts, yield, d, false);
+    if (rc) return rc;
+    void *mem = nullptr;
+    const size_t total = (size_t)offsets[opts->n];
+    const size_t bytes = total * 8 + (size_t)(opts->n + 1) * 8 + T_NCOUNT * 4;
+    if (hipMalloc(&mem, bytes) != hipSuccess) return fail(HWT_ENOMEM, "hipMalloc alphas failed");
+    double *dal = (double *)mem;
+    int64_t *doff = (int64_t *)(dal + total);
+    int32_t *tally = (int32_t *)(doff + opts->n + 1);
+    rc = HWT_OK;
+    if (hipMemcpy(doff, offsets, (size_t)(opts->n + 1) * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(tally, 0, T_NCOUNT * 4) != hipSuccess)
+        rc = fail(HWT_EHIP, "hwt_alphas_f64: setup copies failed");
+    if (!rc) {
+        k_alphas<<<opts->n, WAVE>>>(dev_opts(d.o), DevYield{d.y.n, d.y.rate, d.y.day}, doff, tally, dal);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(HWT_EHIP, "hwt_alphas_f64: kernel failed");
+        else if (hipMemcpy(alphas, dal, total * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(HWT_EHIP, "hwt_alphas_f64: D2H failed");
+    }
+    (void)hipFree(mem);
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,154 @@
+/*
+ * hwtree.h -- C ABI of the MI355X-native Hull-White one-factor trinomial-tree
+ * pricer (European options on zero-coupon bonds).
+ *
+ * This is the drop-in boundary for the reference's price-vector entry points:
+ *
+ *   void seq::computeOptions(const Options&, const Yield&, std::vector<real>&)
+ *        -- reference seq/Seq.hpp:181-190
+ *   void cuda::option::KernelRunBase::run(const Options&, const Yield&,
+ *        std::vector<real>&, int blockSize, SortType, bool isTest)
+ *        -- reference cuda-option/Kernel.cuh:303-314
+ *   void cuda::multi::KernelRunBase::run(...same...)
+ *        -- reference cuda-multi/Kernel.cuh:374-403
+ *
+ * Inputs are the reference's structure-of-arrays trinom::Options
+ * (common/Options.hpp:48-58, same fields, order and element types) and
+ * trinom::Yield (common/Yield.hpp:14-18).  Prices come back in input order.
+ * Nothing here throws; every call returns 0 or a negative HWT_E* code.
+ * The caller owns every buffer.  A plan is bound to one device and is not
+ * safe to share between host threads.
+ */
+#ifndef HWTREE_H
+#define HWTREE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HWT_ABI_VERSION 1
+
+/* Return codes. */
+#define HWT_OK            0
+#define HWT_EINVAL       -1  /* bad argument: null pointer, n < 0, bad type, a <= 0, ... */
+#define HWT_EUNSUPPORTED -2  /* valid option the selected kernel form cannot price (e.g. width) */
+#define HWT_ENOMEM       -3  /* device or host allocation failed, or capacity exceeded */
+#define HWT_EHIP         -4  /* HIP runtime error */
+
+/* Kernel forms (the reference's two CUDA runners plus an inspector). */
+#define HWT_FORM_AUTO 0  /* per option: lane form for tiny trees, wave form otherwise */
+#define HWT_FORM_LANE 1  /* form 1: one lane per option, columns interleaved in HBM
+                            (replaces cuda-option, reference cuda-option/Kernel.cuh:65-216) */
+#define HWT_FORM_WAVE 2  /* form 2: columns on chip, many options per workgroup
+                            (replaces cuda-multi, reference cuda-multi/Kernel.cuh:60-292) */
+
+/* Precision modes. */
+#define HWT_PREC_FP64  0 /* all tree arithmetic in FP64 */
+#define HWT_PREC_MIXED 1 /* Q propagation and backward induction in FP32; alpha
+                            calibration sums, logs and yield interpolation in FP64 */
+
+/* Processing-order hint, the reference's SortType (common/Options.hpp:15-22).
+ * Results are always returned in input order. */
+#define HWT_SORT_NONE        '-'
+#define HWT_SORT_WIDTH_ASC   'w'
+#define HWT_SORT_WIDTH_DESC  'W'
+#define HWT_SORT_HEIGHT_ASC  'h'
+#define HWT_SORT_HEIGHT_DESC 'H'
+#define HWT_SORT_WORK_DESC   'L'  /* longest-processing-time first (default) */
+
+/* trinom::Options, FP64 (common/Options.hpp:48-58).  Pointers are host
+ * pointers for hwt_price_f64 and device pointers for hwt_plan_execute. */
+typedef struct {
+    int32_t n;
+    const double   *strike;      /* StrikePrices   */
+    const double   *maturity;    /* Maturities: bond maturity in years, sets the tree height */
+    const double   *length;      /* Lengths: option expiry in years, sets the exercise step */
+    const uint16_t *term_unit;   /* TermUnits (days per term unit, 365 = yearly) */
+    const uint16_t *term_steps;  /* TermStepCounts (steps per term unit) */
+    const double   *a;           /* ReversionRates */
+    const double   *sigma;       /* Volatilities   */
+    const int8_t   *type;        /* Types: CALL = 0, PUT = 1 */
+} hwt_options_f64;
+
+/* trinom::Yield (common/Yield.hpp:14-18): continuously compounded zero rates
+ * at day offsets, ascending. */
+typedef struct {
+    int32_t n;
+    const double  *rate;         /* Prices    */
+    const int32_t *day;          /* TimeSteps */
+} hwt_yield_f64;
+
+typedef struct {
+    int32_t form;        /* HWT_FORM_*                                  (0 = auto) */
+    int32_t precision;   /* HWT_PREC_*                                  (0 = FP64) */
+    int32_t block_size;  /* form 1: lanes per workgroup (64..1024); form 2: waves per
+                            workgroup x 64.  <= 0 picks the tuned default. */
+    int32_t sort;        /* HWT_SORT_* (0 = work-descending) */
+    int32_t device;      /* HIP device ordinal for hwt_price_f64; < 0 = current */
+    int32_t flags;       /* HWT_FLAG_* */
+    int32_t reserved[2];
+} hwt_config;
+
+/* Launch the plan's kernels one by one instead of replaying its HIP graph;
+ * hwt_stats.kernel_ms then times the pricing kernel alone. */
+#define HWT_FLAG_NO_GRAPH 1
+
+typedef struct {
+    double  kernel_ms;        /* pricing kernel alone (HWT_FLAG_NO_GRAPH), else = total_ms */
+    double  total_ms;         /* whole plan execution: constants, ordering, pricing;
+                                 excludes H2D/D2H copies.  HIP events. */
+    int64_t device_memory;    /* bytes of device memory held by the plan */
+    int64_t node_updates;     /* stencil node updates performed (active tree region) */
+    int64_t seq_node_updates; /* node updates the reference's seq would perform (both passes) */
+    int32_t tasks;            /* work items scheduled */
+    int32_t unsupported;      /* options that could not be priced (output NaN) */
+} hwt_stats;
+
+/* Fills *cfg with defaults (auto form, FP64, tuned sizes, work-descending order). */
+void hwt_config_init(hwt_config *cfg);
+
+/* One-shot pricing of host arrays; replaces seq::computeOptions and
+ * KernelRunBase::run.  out_prices (host, n doubles) receives prices in input
+ * order.  cfg and stats may be NULL. */
+int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
+                  double *out_prices, const hwt_config *cfg, hwt_stats *stats);
+
+/* Reusable plan over device-resident inputs (no allocation per execution;
+ * the pipeline is captured into a HIP graph).  capacity = max options per
+ * execution, yield_capacity = max yield knots. */
+typedef struct hwt_plan hwt_plan;
+int  hwt_plan_create(hwt_plan **plan, int32_t capacity, int32_t yield_capacity,
+                     const hwt_config *cfg);
+/* Enqueue pricing of device arrays on `stream` (a hipStream_t, NULL = the
+ * plan's own stream).  Asynchronous: call hwt_plan_sync for the status. */
+int  hwt_plan_execute(hwt_plan *plan, const hwt_options_f64 *dev_opts,
+                      const hwt_yield_f64 *dev_yield, double *dev_out, void *stream);
+/* Wait for the last execution; returns its status (HWT_EUNSUPPORTED if any
+ * option could not be priced) and fills stats if non-NULL. */
+int  hwt_plan_sync(hwt_plan *plan, hwt_stats *stats);
+void hwt_plan_destroy(hwt_plan *plan);
+
+/* Per-step alpha calibration of each option (alphas[0..n] of the reference's
+ * forward pass, seq/Seq.hpp:56-106), host arrays.  alpha_offsets (n_opts+1
+ * int64, host) is filled first when alphas == NULL so the caller can size the
+ * buffer; alphas (host, alpha_offsets[n_opts] doubles) then receives the
+ * sequences back to back. */
+int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
+                   int64_t *alpha_offsets, double *alphas);
+
+/* Largest tree width (2*jmax+1) each form supports. */
+int32_t hwt_max_width(int32_t form);
+
+const char *hwt_strerror(int code);
+/* Detail of the last error on the calling thread ("" if none). */
+const char *hwt_last_error(void);
+int32_t hwt_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HWTREE_H */

@@ -1,0 +1,61 @@
+"""Write the committed golden fixtures from the reference's own data files.
+
+Runs only in the build container, where the reference checkout exists
+(default /root/reference, or $HWTREE_REFERENCE).  It copies data, never code:
+each fixture is an input file in the reference's text format
+(common/Options.hpp:87-94, common/Yield.hpp:34-35, common/Arrays.hpp:15-46)
+and the matching FP64 golden output written by the reference's seq binary
+(seq/compute.sh:8,18,33 -> data/out/).  Tokens are copied verbatim so the
+doubles parse to exactly the values the reference read.
+
+Slices: the small files are taken whole; each 65,536-option set is sliced to
+every 64th option (1,024 rows), the 60,000-option set to every 60th.
+"""
+import os
+import sys
+from pathlib import Path
+
+REF = Path(os.environ.get("HWTREE_REFERENCE", "/root/reference"))
+OUT = Path(__file__).resolve().parent
+
+WHOLE = ["book", "options-1000", "0_UNIFORM_1000", "1_RAND_1000",
+         "2_RANDCONSTHEIGHT_1000", "3_RANDCONSTWIDTH_1000"]
+SLICED = {"4_SKEWED": 64, "5_SKEWEDCONSTHEIGHT": 64, "6_SKEWEDCONSTWIDTH": 64,
+          "options-60000": 60}
+
+
+def read_token_lines(path):
+    lines = []
+    for raw in path.read_text().splitlines():
+        raw = raw.strip()
+        if not raw:
+            continue
+        assert raw[0] == "[" and raw[-1] == "]", path
+        lines.append([t.strip() for t in raw[1:-1].split(",")])
+    return lines
+
+
+def write_token_lines(path, lines):
+    path.write_text("".join("[" + ", ".join(toks) + "]\n" for toks in lines))
+
+
+def main():
+    if not (REF / "data").is_dir():
+        sys.exit(f"reference data not found under {REF}")
+    (OUT / "yield.in").write_text((REF / "data" / "yield.in").read_text())
+    for name in WHOLE:
+        write_token_lines(OUT / f"{name}.in", read_token_lines(REF / "data" / f"{name}.in"))
+        write_token_lines(OUT / f"{name}.out", read_token_lines(REF / "data" / "out" / f"{name}.out"))
+    for name, step in SLICED.items():
+        src_in = read_token_lines(REF / "data" / f"{name}.in")
+        src_out = read_token_lines(REF / "data" / "out" / f"{name}.out")
+        n = len(src_out[0])
+        assert all(len(t) == n for t in src_in), name
+        keep = list(range(0, n, step))
+        write_token_lines(OUT / f"{name}_s{step}.in", [[t[i] for i in keep] for t in src_in])
+        write_token_lines(OUT / f"{name}_s{step}.out", [[src_out[0][i] for i in keep]])
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()
