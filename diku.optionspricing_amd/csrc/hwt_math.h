@@ -107,37 +107,49 @@ HWT_HD double branch_prob(const TreeShape &t, int r, int s)
     return 0.0;
 }
 
-HWT_HD double op_F(const TreeShape &t, int r, int s)
+// Node discounts E_k come from `disc(k)`, so callers can supply a table.
+struct ExactDisc {
+    const TreeShape &t;
+    HWT_HD double operator()(int k) const { return node_disc(k, t.jmax, t.dr, t.dt); }
+};
+
+template <class Disc>
+HWT_HD double op_F(const TreeShape &t, const Disc &disc, int r, int s)
 {
     if (r < 0 || s < 0 || r >= t.w || s >= t.w) return 0.0;
     const double p = branch_prob(t, r, s);
-    return p == 0.0 ? 0.0 : p * node_disc(s, t.jmax, t.dr, t.dt);
+    return p == 0.0 ? 0.0 : p * disc(s);
 }
 
-HWT_HD double op_FTinv(const TreeShape &t, int r, int s)
+template <class Disc>
+HWT_HD double op_FTinv(const TreeShape &t, const Disc &disc, int r, int s)
 {
-    double v = op_F(t, r, s);
-    if (s == 0) v -= t.rho * op_F(t, r, 1);
-    if (s == t.w - 1) v -= t.rhop * op_F(t, r, t.w - 2);
+    double v = op_F(t, disc, r, s);
+    if (s == 0) v -= t.rho * op_F(t, disc, r, 1);
+    if (s == t.w - 1) v -= t.rhop * op_F(t, disc, r, t.w - 2);
     return v;
 }
 
 // G[r][s] of the tridiagonal operator.
-HWT_HD double op_G(const TreeShape &t, int r, int s)
+template <class Disc>
+HWT_HD double op_G(const TreeShape &t, const Disc &disc, int r, int s)
 {
     if (r < 0 || s < 0 || r >= t.w || s >= t.w) return 0.0;
-    double v = op_FTinv(t, r, s);
-    if (r == 1) v += t.rho * op_FTinv(t, 0, s);
-    if (r == t.w - 2) v += t.rhop * op_FTinv(t, t.w - 1, s);
+    double v = op_FTinv(t, disc, r, s);
+    if (r == 1) v += t.rho * op_FTinv(t, disc, 0, s);
+    if (r == t.w - 2) v += t.rhop * op_FTinv(t, disc, t.w - 1, s);
     return v;
 }
 
+HWT_HD double op_G(const TreeShape &t, int r, int s) { return op_G(t, ExactDisc{t}, r, s); }
+
 HWT_HD void edge_factors(TreeShape &t)
 {
+    const ExactDisc d{t};
     t.rho = 0.0;
     t.rhop = 0.0;
-    t.rho = op_F(t, 2, 0) / op_F(t, 2, 1);
-    t.rhop = op_F(t, t.w - 3, t.w - 1) / op_F(t, t.w - 3, t.w - 2);
+    t.rho = op_F(t, d, 2, 0) / op_F(t, d, 2, 1);
+    t.rhop = op_F(t, d, t.w - 3, t.w - 1) / op_F(t, d, t.w - 3, t.w - 2);
 }
 
 // ------------------------------------------------------------- work counts

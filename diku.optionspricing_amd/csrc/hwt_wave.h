@@ -7,7 +7,8 @@
 //     z'[k] = cl[k] z[k-1] + cm[k] z[k] + cr[k] z[k+1]
 // with per-node coefficients held in registers; the two values that cross a
 // lane boundary move with DPP wave shifts (wave_shr:1 / wave_shl:1), so a step
-// costs 3K FP64 FMA-class ops plus 4 DPP moves per lane and touches no memory.
+// costs 3K FP64 ops (mul, fma, fma) plus 4 DPP moves per lane and touches no
+// memory.
 //
 // Phases.  The active part of the tree grows by one node per side per forward
 // step until it reaches the edges, and the part of the backward pass that can
@@ -25,28 +26,134 @@ constexpr int WAVE = 64;
 constexpr int MAXK = 21;                 // widths up to 64 * 21 = 1344 nodes
 constexpr int WAVE_MAX_WIDTH = WAVE * MAXK;
 
+typedef __attribute__((address_space(3))) double lds_f64;
+
 // ------------------------------------------------------------------- DPP
-__device__ __forceinline__ double dpp_from_lower_lane(double x)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
 {
-    // lane l receives lane l-1; lane 0 receives 0 (bound_ctrl)
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, true);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double dpp_from_upper_lane(double x)
+// lane l receives lane l-1; lane 0 receives 0
+__device__ __forceinline__ double dpp_from_lower_lane(double x) { return dpp_mov<0x138>(x); }
+// lane l receives lane l+1; lane 63 receives 0
+__device__ __forceinline__ double dpp_from_upper_lane(double x) { return dpp_mov<0x130>(x); }
+
+__device__ __forceinline__ double readlane_f64(double x, int lane)
 {
-    // lane l receives lane l+1; lane 63 receives 0 (bound_ctrl)
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                            __builtin_amdgcn_readlane(__double2loint(x), lane));
 }
 
+// Sum over the 64 lanes, the same in every lane: DPP butterflies inside each
+// 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], half-row mirror, row
+// mirror), then the four row sums are read out and added in a fixed order.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Scalar (wave-uniform) copies of values the caller guarantees are uniform.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v)
+{
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+__device__ __forceinline__ TreeShape uni(const TreeShape &t)
+{
+    return TreeShape{uni(t.w), uni(t.jmax), uni(t.M), uni(t.dr), uni(t.dt), uni(t.rho), uni(t.rhop)};
+}
+
+// ------------------------------------------------------------ edge table
+// For each pass (0 forward, 1 backward) the three-point coefficients of nodes
+// 0, 1, 2, w-3, w-2, w-1 -- every node where the edge-corrected operator
+// differs from the closed forms -- 36 entries.  Computed once per option by the
+// setup kernel and handed to the pricing wave through LDS.
+constexpr int EDGE_TAB = 2 * 6 * 3;
+
+HWT_HD int edge_node(int slot, int w) { return slot < 3 ? slot : w - 6 + slot; }
+
+// Node discounts of nodes 0..3 and w-4..w-1 (all the edge coefficients need).
+struct EdgeDisc {
+    double E[8];
+    int w;
+    HWT_HD double operator()(int k) const { return k < 4 ? E[k] : E[4 + (k - (w - 4))]; }
+};
+
+// General form: every entry from the definition G = T F T^{-1}.
+HWT_HD void edge_table_general(const TreeShape &t, double *tab)
+{
+    EdgeDisc d;
+    d.w = t.w;
+    for (int i = 0; i < 8; ++i) {
+        const int k = i < 4 ? i : t.w - 8 + i;
+        d.E[i] = (k >= 0 && k < t.w) ? node_disc(k, t.jmax, t.dr, t.dt) : 0.0;
+    }
+    for (int slot = 0; slot < 6; ++slot) {
+        const int k = edge_node(slot, t.w);
+        for (int which = 0; which < 3; ++which) {
+            tab[3 * slot + which] = op_G(t, d, k, k - 1 + which);       // forward: G[k][k-1..k+1]
+            tab[18 + 3 * slot + which] = op_G(t, d, k - 1 + which, k);  // backward: G^T[k][k-1..k+1]
+        }
+    }
+}
+
+// Closed form for trees at least 7 nodes wide, where the two edge
+// corrections touch disjoint rows and columns.  With F[r][s] = p(s->r) E_s,
+// bottom rho = F20/F21, top rhop = F[c][a]/F[c][b] (a = w-1, b = w-2, c = w-3):
+//   G00 = F00 - rho F01,  G10 = F10 - rho F11 + rho G00,  G11 = F11 + rho F01,
+//   G[a][a] = Faa - rhop Fab,  G[b][a] = Fba - rhop Fbb + rhop G[a][a],
+//   G[b][b] = Fbb + rhop Fab,  every other entry equals F.
+HWT_HD void edge_table(const TreeShape &t, double *tab)
+{
+    if (t.w < 7) {
+        edge_table_general(t, tab);
+        return;
+    }
+    const double M = t.M, rho = t.rho, rhop = t.rhop;
+    const int J = t.jmax, a = t.w - 1, b = t.w - 2, c = t.w - 3, d = t.w - 4;
+    const double E0 = node_disc(0, J, t.dr, t.dt), E1 = node_disc(1, J, t.dr, t.dt);
+    const double E2 = node_disc(2, J, t.dr, t.dt), E3 = node_disc(3, J, t.dr, t.dt);
+    const double Ea = node_disc(a, J, t.dr, t.dt), Eb = node_disc(b, J, t.dr, t.dt);
+    const double Ec = node_disc(c, J, t.dr, t.dt), Ed = node_disc(d, J, t.dr, t.dt);
+    // bottom: node 0 branches (j=-J) to 0,1,2; nodes 1,2,3 are standard
+    const double F00 = pd_bot(-J, M) * E0, F10 = pm_bot(-J, M) * E0;
+    const double F01 = pd_std(1 - J, M) * E1, F11 = pm_std(1 - J, M) * E1, F21 = pu_std(1 - J, M) * E1;
+    const double F12 = pd_std(2 - J, M) * E2, F22 = pm_std(2 - J, M) * E2, F32 = pu_std(2 - J, M) * E2;
+    const double F23 = pd_std(3 - J, M) * E3;
+    const double G00 = F00 - rho * F01;
+    const double G10 = F10 - rho * F11 + rho * G00;
+    const double G11 = F11 + rho * F01;
+    // top: node a branches (j=J) to a,b,c; nodes b,c,d are standard
+    const double Faa = pu_top(J, M) * Ea, Fba = pm_top(J, M) * Ea;
+    const double Fab = pu_std(b - J, M) * Eb, Fbb = pm_std(b - J, M) * Eb, Fcb = pd_std(b - J, M) * Eb;
+    const double Fbc = pu_std(c - J, M) * Ec, Fcc = pm_std(c - J, M) * Ec, Fdc = pd_std(c - J, M) * Ec;
+    const double Fcd = pu_std(d - J, M) * Ed;
+    const double Gaa = Faa - rhop * Fab;
+    const double Gba = Fba - rhop * Fbb + rhop * Gaa;
+    const double Gbb = Fbb + rhop * Fab;
+    const double fwd[18] = {0.0, G00, F01,  G10, G11, F12,  F21, F22, F23,
+                            Fcd, Fcc, Fcb,  Fbc, Gbb, Gba,  Fab, Gaa, 0.0};
+    const double bwd[18] = {0.0, G00, G10,  F01, G11, F21,  F12, F22, F32,
+                            Fdc, Fcc, Fbc,  Fcb, Gbb, Fab,  Gba, Gaa, 0.0};
+    for (int i = 0; i < 18; ++i) {
+        tab[i] = fwd[i];
+        tab[18 + i] = bwd[i];
+    }
+}
+
+__device__ __forceinline__ void load_edge_table(const double *src, int lane, lds_f64 *tab)
+{
+    if (lane < EDGE_TAB) tab[lane] = src[lane];
+    __syncthreads();
 }
 
 // ------------------------------------------------------------- stencil
@@ -85,93 +192,74 @@ __device__ __forceinline__ void stencil_steps(double (&z)[K], const double (&cl)
     }
 }
 
-typedef __attribute__((address_space(3))) double lds_f64;
-
-// Scalar (wave-uniform) copies of values the caller guarantees are uniform.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uni(double v)
-{
-    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
-                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
-}
-__device__ __forceinline__ TreeShape uni(const TreeShape &t)
-{
-    return TreeShape{uni(t.w), uni(t.jmax), uni(t.M), uni(t.dr), uni(t.dt), uni(t.rho), uni(t.rhop)};
-}
-
-// Edge coefficient table: for each pass (0 forward, 1 backward) the
-// three-point coefficients of nodes 0, 1, 2, w-3, w-2, w-1, i.e. every node
-// where the edge-corrected operator differs from the closed forms.
-constexpr int EDGE_TAB = 2 * 6 * 3;
-
-__device__ __forceinline__ int edge_slot(int k, int w) { return k < 3 ? k : 3 + (k - (w - 3)); }
-
-// Fills the table; lanes 0..35 each compute one entry.
-__device__ __forceinline__ void fill_edge_table(const TreeShape &t, int lane, lds_f64 *tab)
-{
-    if (lane < EDGE_TAB) {
-        const int pass = lane / 18, slot = (lane % 18) / 3, which = lane % 3;
-        const int k = slot < 3 ? slot : t.w - 3 + (slot - 3);
-        double v;
-        if (pass == 0) v = op_G(t, k, k - 1 + which);             // forward: G[k][k-1..k+1]
-        else v = op_G(t, k - 1 + which, k);                        // backward: G^T[k][k-1..k+1]
-        tab[lane] = v;
-    }
-    __syncthreads();
-}
-
-// One coefficient of the three-point stencil (which: 0 left, 1 mid, 2 right)
-// for every node of the window [base, base + 64K), written to buf[0, 64K) in
-// node order.  Lanes take nodes base+lane, base+lane+64, ...; the node
-// discounts E_k = e^{-(k-jmax) dr dt} advance by e^{-64 dr dt} per node.
-// Interior nodes use the closed forms; nodes next to the edges read the
-// precomputed table; padding beyond the top edge is zero.
-template <bool FWD>
-__device__ __forceinline__ void stage_coef(const TreeShape &t, int base, int K, int lane, int which, lds_f64 *buf,
-                                           const lds_f64 *etab)
-{
-    const int d = FWD ? which - 1 : 0;  // forward coefficients use the source node's discount
-    const double q64 = exp(-64.0 * t.dr * t.dt);
-    double E = node_disc(base + lane + d, t.jmax, t.dr, t.dt);
-    const int nwin = WAVE * K;
-#pragma unroll 1
-    for (int m = lane; m < nwin; m += WAVE) {
-        const int k = base + m;
-        const int j = k - t.jmax;
-        double v;
-        if (k >= t.w) {
-            v = 0.0;
-        } else if (k < 3 || k > t.w - 4) {
-            v = etab[(FWD ? 0 : 18) + 3 * edge_slot(k, t.w) + which];
-        } else if (FWD) {
-            v = (which == 0 ? pu_std(j - 1, t.M) : which == 1 ? pm_std(j, t.M) : pd_std(j + 1, t.M)) * E;
-        } else {
-            v = (which == 0 ? pd_std(j, t.M) : which == 1 ? pm_std(j, t.M) : pu_std(j, t.M)) * E;
-        }
-        buf[m] = v;
-        E *= q64;
-    }
-    __syncthreads();
-}
-
-template <int K>
-__device__ __forceinline__ void fetch_coef(int lane, const lds_f64 *buf, double (&c)[K])
-{
-#pragma unroll
-    for (int s = 0; s < K; ++s) c[s] = buf[lane * K + s];
-    __syncthreads();
-}
-
+// Coefficients of this lane's K nodes for the forward operator G (FWD) or the
+// backward operator G^T.  Interior nodes use the closed forms with the node
+// discounts E_k = e^{-(k-jmax) dr dt} advanced by one multiplication per node;
+// padding beyond the top edge is zero; the (at most six) edge nodes inside the
+// window are then overwritten from the table, with a wave-uniform switch on
+// their slot so every register index stays static.
 template <int K, bool FWD>
-__device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int lane, lds_f64 *buf, const lds_f64 *etab,
+__device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int lane, const lds_f64 *etab,
                                              double (&cl)[K], double (&cm)[K], double (&cr)[K])
 {
-    stage_coef<FWD>(t, base, K, lane, 0, buf, etab);
-    fetch_coef<K>(lane, buf, cl);
-    stage_coef<FWD>(t, base, K, lane, 1, buf, etab);
-    fetch_coef<K>(lane, buf, cm);
-    stage_coef<FWD>(t, base, K, lane, 2, buf, etab);
-    fetch_coef<K>(lane, buf, cr);
+    const int k0 = base + lane * K;
+    const double q = exp(-t.dr * t.dt);
+    if (FWD) {
+        // outgoing weights pu*E (to k+1), pm*E (to k), pd*E (to k-1) of the
+        // source nodes k0-1 .. k0+K, rolled into the targets' three slots
+        double E = node_disc(k0 - 1, t.jmax, t.dr, t.dt);
+#pragma unroll
+        for (int m = 0; m < K + 2; ++m) {
+            const int k = k0 - 1 + m;
+            const int j = k - t.jmax;
+            const double Ek = (k >= 0 && k < t.w) ? E : 0.0;
+            if (m < K) cl[m < K ? m : 0] = pu_std(j, t.M) * Ek;
+            if (m >= 1 && m <= K) cm[(m >= 1 && m <= K) ? m - 1 : 0] = pm_std(j, t.M) * Ek;
+            if (m >= 2) cr[m >= 2 ? m - 2 : 0] = pd_std(j, t.M) * Ek;
+            E *= q;
+        }
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+            if (k0 + s >= t.w) cl[s] = cm[s] = cr[s] = 0.0;
+    } else {
+        double E = node_disc(k0, t.jmax, t.dr, t.dt);
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const int j = k0 + s - t.jmax;
+            const double Ek = (k0 + s < t.w) ? E : 0.0;
+            cl[s] = pd_std(j, t.M) * Ek;
+            cm[s] = pm_std(j, t.M) * Ek;
+            cr[s] = pu_std(j, t.M) * Ek;
+            E *= q;
+        }
+    }
+    const lds_f64 *tab = etab + (FWD ? 0 : 18);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int k = edge_node(e, t.w);
+        const int off = k - base;
+        if (k >= 0 && k < t.w && off >= 0 && off < WAVE * K) {
+            const int L = off / K, S = off - L * K;  // wave-uniform
+            const double a = tab[3 * e], b = tab[3 * e + 1], c = tab[3 * e + 2];
+            if (lane == L) {
+                switch (S) {
+#define HWT_SLOT(SS)                                       \
+    case SS:                                               \
+        if (SS < K) {                                      \
+            cl[SS < K ? SS : 0] = a;                       \
+            cm[SS < K ? SS : 0] = b;                       \
+            cr[SS < K ? SS : 0] = c;                       \
+        }                                                  \
+        break;
+                    HWT_SLOT(0) HWT_SLOT(1) HWT_SLOT(2) HWT_SLOT(3) HWT_SLOT(4) HWT_SLOT(5) HWT_SLOT(6)
+                    HWT_SLOT(7) HWT_SLOT(8) HWT_SLOT(9) HWT_SLOT(10) HWT_SLOT(11) HWT_SLOT(12) HWT_SLOT(13)
+                    HWT_SLOT(14) HWT_SLOT(15) HWT_SLOT(16) HWT_SLOT(17) HWT_SLOT(18) HWT_SLOT(19) HWT_SLOT(20)
+#undef HWT_SLOT
+                default: break;
+                }
+            }
+        }
+    }
 }
 
 // Initial values: forward delta at the root node; backward T^{-T} * 100.
@@ -195,13 +283,11 @@ __device__ __forceinline__ void init_window(double (&z)[K], const TreeShape &t, 
 
 // One phase: bring the window in (initial values, or the previous window's
 // nodes [in_lo, in_hi) from LDS, zeros elsewhere), run nsteps stencil steps,
-// and leave the window's tree nodes in LDS for the next phase.
+// and leave the window's tree nodes in LDS, or in `out` when given.
 template <int K, bool FWD>
-__device__ __noinline__ void phase_fn(TreeShape tv, int basev, int nstepsv, int in_lov, int in_hiv, int lane,
-                                      lds_f64 *lds, const lds_f64 *etab)
+__device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nsteps, int in_lo, int in_hi, int lane,
+                                         lds_f64 *lds, const lds_f64 *etab, double *out)
 {
-    const TreeShape t = uni(tv);
-    const int base = uni(basev), nsteps = uni(nstepsv), in_lo = uni(in_lov), in_hi = uni(in_hiv);
     double z[K];
     if (in_lo >= in_hi) {
         init_window<K, FWD>(z, t, base, lane);
@@ -215,13 +301,21 @@ __device__ __noinline__ void phase_fn(TreeShape tv, int basev, int nstepsv, int 
     __syncthreads();
     if (nsteps > 0) {
         double cl[K], cm[K], cr[K];
-        window_coefs<K, FWD>(t, base, lane, lds, etab, cl, cm, cr);
+        window_coefs<K, FWD>(t, base, lane, etab, cl, cm, cr);
         stencil_steps<K>(z, cl, cm, cr, nsteps);
     }
+    if (out) {
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-        const int k = base + lane * K + s;
-        if (k < t.w) lds[k] = z[s];
+        for (int s = 0; s < K; ++s) {
+            const int k = base + lane * K + s;
+            if (k < t.w) out[k] = z[s];
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const int k = base + lane * K + s;
+            if (k < t.w) lds[k] = z[s];
+        }
     }
     __syncthreads();
 }
@@ -229,12 +323,16 @@ __device__ __noinline__ void phase_fn(TreeShape tv, int basev, int nstepsv, int 
 #define HWT_KCASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) \
     X(14) X(15) X(16) X(17) X(18) X(19) X(20) X(21)
 
-template <bool FWD>
+template <int KMAX, bool FWD>
 __device__ __forceinline__ void dispatch_phase(int K, const TreeShape &t, int base, int nsteps, int in_lo, int in_hi,
-                                               int lane, lds_f64 *lds, const lds_f64 *etab)
+                                               int lane, lds_f64 *lds, const lds_f64 *etab, double *out)
 {
     switch (K) {
-#define HWT_C(KK) case KK: phase_fn<KK, FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab); break;
+#define HWT_C(KK)                                                                                     \
+    case KK:                                                                                          \
+        if (KK <= KMAX)                                                                               \
+            phase_fn<(KK <= KMAX ? KK : 1), FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab, out); \
+        break;
         HWT_KCASES(HWT_C)
 #undef HWT_C
     default: break;
@@ -254,10 +352,11 @@ __device__ __forceinline__ Window window_for(int D, const TreeShape &t)
     return Window{(t.w + WAVE - 1) / WAVE, 0};
 }
 
-// Forward pass: state 0 (delta at the root) -> state e.  The state is left in
-// lds[lo, hi); tree nodes outside that range are zero.
+// Forward pass: state 0 (delta at the root) -> state e.  The state ends up in
+// out[lo, hi) (global scratch); tree nodes outside that range are zero.
+template <int KMAX>
 __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, lds_f64 *lds, const lds_f64 *etab,
-                                             int &lo, int &hi)
+                                             double *out, int &lo, int &hi)
 {
     Window win = window_for(0, t);
     int in_lo = 0, in_hi = 0;  // empty: initial values
@@ -265,11 +364,12 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
     for (;;) {
         const bool full = 32 * win.K > t.jmax;  // growth windows satisfy 32K <= jmax
         const int iend = full ? e : min(e, 32 * win.K - 1);
-        dispatch_phase<true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab);
+        const bool last = iend >= e;
+        dispatch_phase<KMAX, true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab, last ? out : nullptr);
         i = iend;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
-        if (i >= e) break;
+        if (last) break;
         win = window_for(i + 1, t);
     }
     lo = in_lo;
@@ -277,7 +377,9 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
 }
 
 // Backward pass: state n (bond face value 100) -> state e, keeping only the
-// nodes that can still reach the exercise nodes |j| <= min(e, jmax).
+// nodes that can still reach the exercise nodes |j| <= min(e, jmax).  The
+// state ends up in lds[lo, hi).
+template <int KMAX>
 __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, lds_f64 *lds,
                                               const lds_f64 *etab, int &lo, int &hi)
 {
@@ -290,7 +392,7 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
         const bool full = 32 * win.K > t.jmax;
         const int klo = full ? Kgmax : win.K - 1;
         const int send = max(e, e + 32 * klo - 1 - ep);
-        dispatch_phase<false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab);
+        dispatch_phase<KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, nullptr);
         s = send;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);

@@ -8,8 +8,11 @@
 //              factors, work estimate; trivial options are priced right here
 //   k_scan     bucket offsets, largest work first (longest-processing-time order)
 //   k_scatter  option indices into the task list
-//   k_price_wave  persistent kernel: each wavefront pulls options off the task
-//              list and prices them with the on-chip tree column (hwt_wave.h)
+//   k_price_wide / k_price_narrow  persistent kernels, launched side by side:
+//              each wavefront pulls options off its part of the task list and
+//              prices them with the on-chip tree column (hwt_wave.h); trees up to
+//              128 nodes wide go to the narrow kernel, which needs far fewer
+//              registers and so runs at higher occupancy
 //
 // Pricing identity.  With Q_e the Arrow-Debreu prices at the exercise step e
 // and B_e the bond values there, the reference's backward induction gives
@@ -29,6 +32,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -53,12 +57,26 @@ struct DevYield {
 };
 
 constexpr int NBUCKET = 256;
-enum Tally { T_QUEUE = 0, T_NTASK = 1, T_INVALID = 2, T_UNSUP = 3, T_WORK = 4, T_SEQWORK = 6, T_NCOUNT = 8 };
+constexpr int KSMALL = 2;  // trees up to 64*KSMALL nodes wide go to the narrow-tree kernel
+enum Tally {
+    T_QUEUE = 0,    // next task of the wide-tree kernel (tasks [0, T_NLARGE))
+    T_QUEUE_S = 1,  // next task of the narrow-tree kernel (tasks [T_NLARGE, T_NTASK))
+    T_NTASK = 2,
+    T_NLARGE = 3,
+    T_INVALID = 4,
+    T_UNSUP = 5,
+    T_WORK = 6,     // int64
+    T_SEQWORK = 8,  // int64
+    T_NCOUNT = 10
+};
 
-__device__ __forceinline__ int bucket_of(int64_t key)
+// Buckets [128, 256) hold wide trees, [0, 128) narrow ones; the descending
+// scan puts every wide tree ahead of every narrow one in the task list.
+__device__ __forceinline__ int bucket_of(int64_t key, bool wide)
 {
-    const int b = (int)(8.0 * log2((double)key + 1.0));
-    return b < 0 ? 0 : (b >= NBUCKET ? NBUCKET - 1 : b);
+    int b = (int)(4.0 * log2((double)key + 1.0));
+    b = b < 0 ? 0 : (b > 127 ? 127 : b);
+    return wide ? 128 + b : b;
 }
 
 __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
@@ -72,8 +90,8 @@ __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 }
 
 __global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, OptConst *__restrict__ cst,
-                        int32_t *__restrict__ bucket, int32_t *__restrict__ hist, int32_t *__restrict__ tally,
-                        double *__restrict__ out)
+                        double *__restrict__ edge, int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
+                        int32_t *__restrict__ tally, double *__restrict__ out)
 {
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
     if (i >= o.n) return;
@@ -103,6 +121,7 @@ __global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, O
             edge_factors(t);
             c.rho = t.rho;
             c.rhop = t.rhop;
+            edge_table(t, edge + (size_t)i * EDGE_TAB);
             c.work = dual_node_updates(c.n, c.e, c.jmax);
             const int64_t seqw = seq_node_updates(c.n, c.jmax);
             atomicAdd((unsigned long long *)&tally[T_WORK], (unsigned long long)c.work);
@@ -114,7 +133,8 @@ __global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, O
             case '-': key = 0; break;
             default: key = c.work; break;
             }
-            b = bucket_of(key);
+            const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
+            b = bucket_of(key, wide);
             atomicAdd(&hist[b], 1);
         }
     }
@@ -122,12 +142,14 @@ __global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, O
     bucket[i] = b;
 }
 
-// Exclusive offsets of the buckets, visited largest key first unless ascending.
+// Exclusive offsets of the buckets: every wide-tree bucket (128..255) before
+// every narrow-tree bucket (0..127), each class largest key first unless an
+// ascending order was asked for.
 __global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ offs, int32_t *__restrict__ tally, int32_t asc)
 {
     __shared__ int32_t s[NBUCKET];
     const int tid = threadIdx.x;
-    const int idx = asc ? tid : NBUCKET - 1 - tid;
+    const int idx = tid < 128 ? (asc ? 128 + tid : 255 - tid) : (asc ? tid - 128 : 255 - tid);
     const int32_t v = hist[idx];
     s[tid] = v;
     __syncthreads();
@@ -138,6 +160,7 @@ __global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ o
         __syncthreads();
     }
     offs[idx] = s[tid] - v;
+    if (tid == 127) tally[T_NLARGE] = s[tid];
     if (tid == NBUCKET - 1) tally[T_NTASK] = s[tid];
 }
 
@@ -150,18 +173,21 @@ __global__ void k_scatter(int32_t n, const int32_t *__restrict__ bucket, const i
     if (b >= 0) order[offs[b] + atomicAdd(&cursor[b], 1)] = i;
 }
 
-__device__ double price_option(const OptConst &c, int lane, lds_f64 *ldsA, lds_f64 *ldsB, lds_f64 *etab)
+template <int KMAX>
+__device__ __forceinline__ double price_option(const OptConst &cv, const double *edge, int lane, lds_f64 *lds,
+                                               lds_f64 *etab, double *fscratch)
 {
-    const TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
-    fill_edge_table(t, lane, etab);
+    const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
+    const int n = uni(cv.n), e = uni(cv.e);
+    load_edge_table(edge, lane, etab);
     int flo, fhi, blo, bhi;
-    forward_pass(t, c.e, lane, ldsA, etab, flo, fhi);
-    backward_pass(t, c.n, c.e, lane, ldsB, etab, blo, bhi);
+    forward_pass<KMAX>(t, e, lane, lds, etab, fscratch, flo, fhi);
+    backward_pass<KMAX>(t, n, e, lane, lds, etab, blo, bhi);
 
-    const int ep = min(c.e, c.jmax);
-    const int klo = c.jmax - ep, khi = c.jmax + ep;
-    auto fy = [&](int k) { return (k >= flo && k < fhi) ? ldsA[k] : 0.0; };
-    auto fb = [&](int k) { return (k >= blo && k < bhi) ? ldsB[k] : 0.0; };
+    const int ep = min(e, t.jmax);
+    const int klo = t.jmax - ep, khi = t.jmax + ep;
+    auto fy = [&](int k) { return (k >= flo && k < fhi) ? fscratch[k] : 0.0; };
+    auto fb = [&](int k) { return (k >= blo && k < bhi) ? lds[k] : 0.0; };
     auto yk = [&](int k) {
         double v = fy(k);
         if (k == 1) v -= t.rho * fy(0);
@@ -182,38 +208,127 @@ __device__ double price_option(const OptConst &c, int lane, lds_f64 *ldsA, lds_f
     }
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
-    const double scale = 100.0 * s1 * c.Pn / (s2 * c.Pe);
+    const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
     double s3 = 0.0;
     for (int k = klo + lane; k <= khi; k += WAVE) {
         const double B = scale * bk(k);
-        double v = (c.type == 1) ? c.X - B : B - c.X;  // PUT : CALL
+        double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
         v = (v < 0.0) ? 0.0 : v;
         if (v == INFINITY) v = DBL_MAX;  // common/Domain.hpp:163-168
         s3 += yk(k) * v;
     }
     s3 = wave_sum(s3);
-    return c.Pe / s1 * s3;
+    __syncthreads();
+    return cv.Pe / s1 * s3;
 }
 
-__global__ __launch_bounds__(WAVE) void k_price_wave(const OptConst *__restrict__ cst, const int32_t *__restrict__ order,
-                                                     int32_t *__restrict__ tally, double *__restrict__ out)
+// Narrow trees (width <= 64*K): the whole tree in one full window, both passes
+// in registers, and the duality sums formed straight from the registers.
+template <int K>
+__device__ __forceinline__ double price_narrow(const OptConst &cv, const double *edge, int lane, lds_f64 *etab)
 {
-    __shared__ double ldsA[WAVE_MAX_WIDTH];
-    __shared__ double ldsB[WAVE_MAX_WIDTH];
+    const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
+    const int n = uni(cv.n), e = uni(cv.e);
+    load_edge_table(edge, lane, etab);
+    double cl[K], cm[K], cr[K], zf[K], zb[K];
+    window_coefs<K, true>(t, 0, lane, etab, cl, cm, cr);
+    init_window<K, true>(zf, t, 0, lane);
+    stencil_steps<K>(zf, cl, cm, cr, e);
+    window_coefs<K, false>(t, 0, lane, etab, cl, cm, cr);
+    init_window<K, false>(zb, t, 0, lane);
+    stencil_steps<K>(zb, cl, cm, cr, n - e);
+
+    // undo the edge similarity: y = T^{-1} zf, b = T^T zb
+    const double fl = dpp_from_lower_lane(zf[K - 1]), fr = dpp_from_upper_lane(zf[0]);
+    const double bl = dpp_from_lower_lane(zb[K - 1]), br = dpp_from_upper_lane(zb[0]);
+    double yv[K], bv[K];
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const int k = lane * K + s;
+        const double fL = s > 0 ? zf[s - 1] : fl, fR = s + 1 < K ? zf[s + 1] : fr;
+        const double bL = s > 0 ? zb[s - 1] : bl, bR = s + 1 < K ? zb[s + 1] : br;
+        double y = zf[s], b = zb[s];
+        if (k == 1) y -= t.rho * fL;
+        if (k == t.w - 2) y -= t.rhop * fR;
+        if (k == 0) b += t.rho * bR;
+        if (k == t.w - 1) b += t.rhop * bL;
+        if (k >= t.w) y = b = 0.0;
+        yv[s] = y;
+        bv[s] = b;
+        s1 += y;
+        s2 += y * b;
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
+    double s3 = 0.0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const double B = scale * bv[s];
+        double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
+        v = (v < 0.0) ? 0.0 : v;
+        if (v == INFINITY) v = DBL_MAX;  // common/Domain.hpp:163-168
+        s3 += yv[s] * v;
+    }
+    s3 = wave_sum(s3);
+    return cv.Pe / s1 * s3;
+}
+
+// Persistent pricing kernels: one single-wave workgroup per resident slot;
+// the waves pull tasks [first, last) off the shared list until it is drained.
+// Each wave takes task first + blockIdx.x without an atomic, then claims
+// further tasks from the queue counter.  Lane 0 stores the finished price and
+// claims the next task in one divergent region; every lane then reads the
+// claim back through a convergent broadcast, so all 64 lanes always leave the
+// loop together.
+template <int KMAX, bool NARROW>
+__device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                                            const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                                            int queue, int first, int last, double *__restrict__ out,
+                                            double *__restrict__ scratch)
+{
+    __shared__ double lds[NARROW ? 1 : WAVE * KMAX];
     __shared__ double etab[EDGE_TAB];
     const int lane = threadIdx.x;
-    const int ntask = __builtin_amdgcn_readfirstlane(tally[T_NTASK]);
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&tally[T_QUEUE], 1);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, WAVE));
-        if (t >= ntask) break;
-        const int o = __builtin_amdgcn_readfirstlane(order[t]);
+    double *fscratch = scratch + (size_t)blockIdx.x * (WAVE * KMAX);
+    const int stride = uni((int)gridDim.x);
+    int t = uni((int)blockIdx.x) + first;
+    while (t < last) {
+        const int o = uni(order[t]);
         const OptConst c = cst[o];
-        const double p = price_option(c, lane, (lds_f64 *)ldsA, (lds_f64 *)ldsB, (lds_f64 *)etab);
-        if (lane == 0) out[o] = p;
-        __syncthreads();
+        const double *eo = edge + (size_t)o * EDGE_TAB;
+        double p;
+        if (NARROW) {
+            const int K = (uni(c.w) + WAVE - 1) / WAVE;
+            p = (K == 1) ? price_narrow<1>(c, eo, lane, (lds_f64 *)etab)
+                         : price_narrow<(KMAX >= 2 ? 2 : 1)>(c, eo, lane, (lds_f64 *)etab);
+        } else {
+            p = price_option<KMAX>(c, eo, lane, (lds_f64 *)lds, (lds_f64 *)etab, fscratch);
+        }
+        int claim = 0;
+        if (lane == 0) {
+            out[o] = p;
+            claim = atomicAdd(&tally[queue], 1);
+        }
+        t = uni(__shfl(claim, 0, WAVE)) + first + stride;
     }
+}
+
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void k_price_wide(
+    const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
+    int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
+{
+    const int nlarge = uni(tally[T_NLARGE]);
+    price_tasks<MAXK, false>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
+}
+
+__global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                                                       const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                                                       double *__restrict__ out)
+{
+    const int nlarge = uni(tally[T_NLARGE]), ntask = uni(tally[T_NTASK]);
+    price_tasks<KSMALL, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
 }
 
 // ------------------------------------------------------------- alphas
@@ -221,11 +336,11 @@ __global__ __launch_bounds__(WAVE) void k_price_wave(const OptConst *__restrict_
 // seq/Seq.hpp:56-106): alpha_i dt = A_i - A_{i-1}, A_i = ln(sum_k E_k y_i[k]) + R(t_{i+1}) t_{i+1}.
 template <int K>
 __device__ __noinline__ void alphas_full(const OptConst c, const DevYield y, int tu, int lane, double *dst,
-                                         lds_f64 *buf, const lds_f64 *etab)
+                                         const lds_f64 *etab)
 {
     const TreeShape t = uni(TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop});
     double z[K], cl[K], cm[K], cr[K], wE[K];
-    window_coefs<K, true>(t, 0, lane, buf, etab, cl, cm, cr);
+    window_coefs<K, true>(t, 0, lane, etab, cl, cm, cr);
     init_window<K, true>(z, t, 0, lane);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -266,12 +381,16 @@ __global__ __launch_bounds__(WAVE) void k_alphas(DevOpts o, DevYield y, const in
     c.rho = t.rho;
     c.rhop = t.rhop;
     __shared__ double etab[EDGE_TAB];
-    __shared__ double buf[WAVE_MAX_WIDTH];
-    fill_edge_table(t, lane, (lds_f64 *)etab);
+    if (lane == 0) {
+        double tab[EDGE_TAB];
+        edge_table(t, tab);
+        for (int k = 0; k < EDGE_TAB; ++k) etab[k] = tab[k];
+    }
+    __syncthreads();
     double *dst = alphas + offsets[i];
     const int K = (c.w + WAVE - 1) / WAVE;
     switch (K) {
-#define HWT_C(KK) case KK: alphas_full<KK>(c, y, o.term_unit[i], lane, dst, (lds_f64 *)buf, (const lds_f64 *)etab); break;
+#define HWT_C(KK) case KK: alphas_full<KK>(c, y, o.term_unit[i], lane, dst, (const lds_f64 *)etab); break;
         HWT_KCASES(HWT_C)
 #undef HWT_C
     default: break;
@@ -315,11 +434,15 @@ struct hwt_plan {
     hipStream_t stream = nullptr;
     int32_t capacity = 0, ycap = 0;
     hwt_config cfg{};
-    int grid_wave = 0;
+    int grid_wide = 0, grid_narrow = 0;
+    hipStream_t side = nullptr;       // narrow-tree kernel runs here, concurrently
+    hipEvent_t fork = nullptr, join = nullptr;
+    double *scratch_wide = nullptr;
     // device workspace, one allocation
     void *ws = nullptr;
     size_t ws_bytes = 0;
     OptConst *cst = nullptr;
+    double *edge = nullptr;
     int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *offs = nullptr, *tally = nullptr;
     int32_t *h_tally = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
@@ -342,11 +465,16 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
         const int blocks = (n + 255) / 256;
-        k_setup<<<blocks, 256, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->bucket, p->hist, p->tally, out);
+        k_setup<<<blocks, 256, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->edge, p->bucket, p->hist, p->tally, out);
         k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
         k_scatter<<<blocks, 256, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
-        k_price_wave<<<p->grid_wave, WAVE, 0, s>>>(p->cst, p->order, p->tally, out);
+        HWT_HIP(hipEventRecord(p->fork, s));
+        HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+        k_price_narrow<<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
+        k_price_wide<<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
+        HWT_HIP(hipEventRecord(p->join, p->side));
+        HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
     }
     HWT_HIP(hipGetLastError());
@@ -408,8 +536,25 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
         return fail(code, msg);
     };
     if (hipGetDevice(&p->device) != hipSuccess) return bail(HWT_EHIP, "hipGetDevice failed");
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess)
         return bail(HWT_EHIP, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->join, hipEventDisableTiming) != hipSuccess)
+        return bail(HWT_EHIP, "hipEventCreate failed");
+    // Persistent grids: as many single-wave workgroups as can be resident.
+    int cus = 0, per_cu_w = 0, per_cu_n = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_w, k_price_wide, WAVE, 0) != hipSuccess || per_cu_w < 1)
+        per_cu_w = 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_n, k_price_narrow, WAVE, 0) != hipSuccess || per_cu_n < 1)
+        per_cu_n = 8;
+    p->grid_wide = per_cu_w * cus;
+    p->grid_narrow = per_cu_n * cus;
+    if (const char *v = getenv("HWT_VERBOSE"); v && *v && *v != '0')
+        fprintf(stderr, "hwtree plan: device %d, %d CUs, wide kernel %d blocks (%d/CU), narrow kernel %d blocks (%d/CU)\n",
+                p->device, cus, p->grid_wide, per_cu_w, p->grid_narrow, per_cu_n);
     const size_t cap = (size_t)std::max(capacity, 1);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
@@ -418,34 +563,31 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
         return at;
     };
     const size_t o_cst = carve(cap * sizeof(OptConst));
+    const size_t o_edge = carve(cap * EDGE_TAB * sizeof(double));
     const size_t o_bucket = carve(cap * sizeof(int32_t));
     const size_t o_order = carve(cap * sizeof(int32_t));
     const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
     const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
     const size_t o_tally = carve(T_NCOUNT * sizeof(int32_t));
+    const size_t o_scw = carve((size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
     p->ws_bytes = off;
     if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) return bail(HWT_ENOMEM, "hipMalloc workspace failed");
     char *base = (char *)p->ws;
     p->cst = (OptConst *)(base + o_cst);
+    p->edge = (double *)(base + o_edge);
     p->bucket = (int32_t *)(base + o_bucket);
     p->order = (int32_t *)(base + o_order);
     p->hist = (int32_t *)(base + o_hist);
     p->cursor = (int32_t *)(base + o_cursor);
     p->offs = (int32_t *)(base + o_offs);
     p->tally = (int32_t *)(base + o_tally);
+    p->scratch_wide = (double *)(base + o_scw);
     if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
         hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess)
         return bail(HWT_EHIP, "hipEventCreate failed");
-    // Persistent grid: as many single-wave workgroups as can be resident.
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_price_wave, WAVE, 0) != hipSuccess || per_cu < 1)
-        per_cu = 4;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus < 1)
-        cus = 256;
-    p->grid_wave = per_cu * cus;
     *out = p;
     (void)rc;
     return HWT_OK;
@@ -461,6 +603,9 @@ void hwt_plan_destroy(hwt_plan *p)
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->evk0) (void)hipEventDestroy(p->evk0);
     if (p->evk1) (void)hipEventDestroy(p->evk1);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    if (p->join) (void)hipEventDestroy(p->join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     if (p->h_tally) (void)hipHostFree(p->h_tally);
     if (p->ws) (void)hipFree(p->ws);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -54,6 +54,15 @@ def main():
         keep = list(range(0, n, step))
         write_token_lines(OUT / f"{name}_s{step}.in", [[t[i] for i in keep] for t in src_in])
         write_token_lines(OUT / f"{name}_s{step}.out", [[src_out[0][i] for i in keep]])
+    # Known-answer table for the tree width: the data generator's pairs of
+    # reversion rate -> width at 12 steps/yr (data-generator/RandomDataGenerator.cpp:54-55).
+    import json
+    import re
+    src = (REF / "data-generator" / "RandomDataGenerator.cpp").read_text()
+    rates = [float(x) for x in re.search(r"reversionRates = \{([^}]*)\}", src).group(1).split(",")]
+    widths = [int(x) for x in re.search(r"widths = \{([^}]*)\}", src).group(1).split(",")]
+    (OUT / "width_table.json").write_text(json.dumps({"term_steps": 12, "term_unit": 365,
+                                                      "a": rates, "width": widths}))
     print("fixtures written to", OUT)
 
 
