@@ -1,0 +1,186 @@
+"""Benchmark: options/s of the FP64 trinomial-tree pricer on one MI355X.
+
+Workload: the 60,000-option skewed batch (90% small trees: bond 1-5y at
+12 steps/yr, a in [0.1,0.2]; 10% large trees: bond 10-30y at 36 steps/yr,
+a in [0.01,0.05], widths up to 1,327 nodes, heights up to 1,080 steps),
+synthetic 120-knot yield curve, seeded inputs already resident in HBM.
+A step prices the whole batch: device-side constants and ordering plus the two
+pricing kernels, replayed from the plan's HIP graph.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the wide-tree
+pricing kernel, timed alone with HIP events on its stream): FP64 vector FLOPs
+of the stencil (5 per node update: one multiply and two fused multiply-adds)
+against MI355X's 78.6 TFLOP/s FP64 peak.  `cpu_baseline` times the reference's
+own sequential pricer (built from its sources into oracle/_ref) on a bounded
+stratified sample on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "options/sec on 1 MI355X: FP64 tree vs surrogate+fallback at max err 1e-3"
+FP64_PEAK_TFLOPS = 78.6
+FLOP_PER_NODE_UPDATE = 5
+
+
+def cpu_baseline(pkg, opts, y):
+    """The reference's seq binary (REAL=64) on a stratified sample: every 20th
+    option of the batch (3,000 options, the 9:1 small/large mix kept), run
+    with one thread per host core, plus a single-thread run on every 100th."""
+    import oracle  # the reference binary lives beside the oracle (oracle/_ref)
+    seq = oracle.REF_SEQ
+    kind = "reference"
+    ncores = os.cpu_count() or 1
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        pkg.formats.write_yield(td / "yield.in", y)
+
+        def run(sample, threads):
+            pkg.formats.write_options(td / "opts.in", sample)
+            env = {**os.environ, "OMP_NUM_THREADS": str(threads)}
+            t0 = time.perf_counter()
+            if seq.exists():
+                subprocess.run([str(seq), "-o", str(td / "opts.in"), "-y", str(td / "yield.in")],
+                               check=True, capture_output=True, env=env)
+            else:
+                oracle.price(sample, y, threads=threads)
+            return time.perf_counter() - t0
+
+        if not seq.exists():
+            kind = "port"
+        multi = opts.take(np.arange(0, opts.N, 20))
+        single = opts.take(np.arange(0, opts.N, 100))
+        t_multi = run(multi, ncores)
+        t_single = run(single, 1)
+    return {"value": multi.N / t_multi, "unit": "options/s", "cores": ncores, "kind": kind,
+            "sample": f"every 20th option of the same 60k batch ({multi.N} options, 9:1 small/large), "
+                      f"reference seq (REAL=64, OpenMP) with {ncores} threads, {t_multi:.2f} s",
+            "single_thread_value": single.N / t_single,
+            "single_thread_sample": f"every 100th option ({single.N}), 1 thread, {t_single:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): options are independent, "
+              "so a multi-GPU run is a disjoint split of the batch; see the sharding plan in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    capi, dm = pkg.capi, pkg.devmem
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed_60k()
+    dev = capi.to_device(opts, y)
+
+    # whole-job throughput: graph replays on a caller stream, HIP events around K steps
+    import ctypes as C
+    stream = C.c_void_p()
+    dm._ck(dm.hip().hipStreamCreate(C.byref(stream)))
+    plan = capi.Plan(opts.N, y.N)
+    for _ in range(args.warmup):
+        plan.execute(dev, stream=stream.value)
+    plan.sync()
+    e0, e1 = dm.Event(), dm.Event()
+    dm.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(dev, stream=stream.value)
+    e1.record(stream)
+    e1.synchronize()
+    st = plan.sync()
+    ms = e0.elapsed_ms(e1)
+    ms_per_step = ms / args.steps
+    plan.close()
+
+    # dominant kernel on its own: the same pipeline launched without the graph,
+    # HIP events around the wide-tree kernel on the stream it runs on
+    plan_k = capi.Plan(opts.N, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+    wide, narrow, kern = [], [], []
+    for i in range(args.warmup + args.steps):
+        plan_k.execute(dev)
+        s = plan_k.sync()
+        if i >= args.warmup:
+            wide.append(s["wide_ms"])
+            narrow.append(s["narrow_ms"])
+            kern.append(s["kernel_ms"])
+    plan_k.close()
+    wide_ms = float(np.mean(wide))
+    wide_flop = FLOP_PER_NODE_UPDATE * s["wide_node_updates"]
+    achieved = wide_flop / (wide_ms * 1e-3) / 1e12
+
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_wide_kernel_skewed_60k.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": opts.N / (ms_per_step * 1e-3),
+        "unit": "options/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded skewed 60k batch, 120-knot linear 1%-4% zero curve)",
+        "config": {
+            "workload": "skewed_60k: 60,000 European zero-coupon-bond options on the Hull-White "
+                        "one-factor trinomial tree, FP64; 54,000 small trees (width <= 47, height <= 60) "
+                        "and 6,000 large (width 267-1,327, height 360-1,080)",
+            "options": opts.N,
+            "precision": "fp64",
+            "pipeline": "device constants + LPT ordering + wide/narrow persistent pricing kernels, one HIP graph",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_price_wide",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel_ms": wide_ms,
+            "flop_per_launch": wide_flop,
+            "flop_model": "5 FP64 FLOP per stencil node update (mul + 2 fma), node updates counted over "
+                          "the active tree region",
+        },
+        "detail": {
+            "node_updates": st["node_updates"],
+            "seq_equivalent_node_updates": st["seq_node_updates"],
+            "wide_tasks": st["wide_tasks"],
+            "narrow_kernel_ms": float(np.mean(narrow)),
+            "pricing_kernels_ms": float(np.mean(kern)),
+            "graph_total_ms_per_step": ms_per_step,
+        },
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        line["cpu_baseline"] = cpu_baseline(pkg, opts, y)
+        line["cpu_baseline"]["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

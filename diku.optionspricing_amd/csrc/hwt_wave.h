@@ -20,6 +20,8 @@
 
 #include "hwt_math.h"
 
+#include <cfloat>
+
 namespace hwt {
 
 constexpr int WAVE = 64;
@@ -80,6 +82,8 @@ __device__ __forceinline__ TreeShape uni(const TreeShape &t)
 constexpr int EDGE_TAB = 2 * 6 * 3;
 
 HWT_HD int edge_node(int slot, int w) { return slot < 3 ? slot : w - 6 + slot; }
+// Table slot of edge node k (k < 3 or k > w-4).
+HWT_HD int edge_slot_of(int k, int w) { return k < 3 ? k : 3 + (k - (w - 3)); }
 
 // Node discounts of nodes 0..3 and w-4..w-1 (all the edge coefficients need).
 struct EdgeDisc {

@@ -23,6 +23,7 @@
 // undiscounted backward state after n-e steps, S1 = sum y_e and S2 = y_e . b_e.
 // So each option costs n stencil steps (e forward + n-e backward) with no
 // per-step reductions, against 2n steps plus n reductions in seq/Seq.hpp.
+#include "hwt_lane.h"
 #include "hwt_wave.h"
 #include "../../include/hwtree.h"
 
@@ -67,7 +68,8 @@ enum Tally {
     T_UNSUP = 5,
     T_WORK = 6,     // int64
     T_SEQWORK = 8,  // int64
-    T_NCOUNT = 10
+    T_WORKW = 10,   // int64, wide trees only
+    T_NCOUNT = 12
 };
 
 // Buckets [128, 256) hold wide trees, [0, 128) narrow ones; the descending
@@ -136,6 +138,7 @@ __global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, O
             const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
             b = bucket_of(key, wide);
             atomicAdd(&hist[b], 1);
+            if (wide) atomicAdd((unsigned long long *)&tally[T_WORKW], (unsigned long long)c.work);
         }
     }
     cst[i] = c;
@@ -331,6 +334,21 @@ __global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restric
     price_tasks<KSMALL, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
 }
 
+// Form 1: one lane per option over every scheduled task (tasks ordered by
+// decreasing work, so the lanes of a wave carry similar trees).
+__global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                             const int32_t *__restrict__ order, const int32_t *__restrict__ tally,
+                             double *__restrict__ out, double *__restrict__ cols)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= tally[T_NTASK]) return;
+    const int o = order[gid];
+    const OptConst c = cst[o];
+    double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)LANE_MAX_WIDTH * WAVE) + (gid % WAVE);
+    out[o] = lane_price(c, edge + (size_t)o * EDGE_TAB, LaneColumn{base},
+                        LaneColumn{base + (size_t)LANE_MAX_WIDTH * WAVE});
+}
+
 // ------------------------------------------------------------- alphas
 // Per-step calibrated alphas of one option (the reference's forward pass,
 // seq/Seq.hpp:56-106): alpha_i dt = A_i - A_{i-1}, A_i = ln(sum_k E_k y_i[k]) + R(t_{i+1}) t_{i+1}.
@@ -438,6 +456,8 @@ struct hwt_plan {
     hipStream_t side = nullptr;       // narrow-tree kernel runs here, concurrently
     hipEvent_t fork = nullptr, join = nullptr;
     double *scratch_wide = nullptr;
+    double *cols = nullptr;  // form 1 tree columns
+    int lane_block = 256;
     // device workspace, one allocation
     void *ws = nullptr;
     size_t ws_bytes = 0;
@@ -446,6 +466,7 @@ struct hwt_plan {
     int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *offs = nullptr, *tally = nullptr;
     int32_t *h_tally = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+    hipEvent_t evw0 = nullptr, evw1 = nullptr, evn0 = nullptr, evn1 = nullptr;
     // captured graph and the arguments it was captured with
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -469,10 +490,21 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
         k_scatter<<<blocks, 256, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
+        if (p->cfg.form == HWT_FORM_LANE) {
+            const int lb = p->lane_block;
+            k_price_lane<<<(n + lb - 1) / lb, lb, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->cols);
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
+            HWT_HIP(hipGetLastError());
+            return HWT_OK;
+        }
         HWT_HIP(hipEventRecord(p->fork, s));
         HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, p->side));
         k_price_narrow<<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, p->side));
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
         k_price_wide<<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evw1, s));
         HWT_HIP(hipEventRecord(p->join, p->side));
         HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
@@ -571,6 +603,13 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
     const size_t o_tally = carve(T_NCOUNT * sizeof(int32_t));
     const size_t o_scw = carve((size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
+    const bool lane = p->cfg.form == HWT_FORM_LANE;
+    if (lane) {
+        const int bs = p->cfg.block_size;
+        p->lane_block = (bs >= WAVE && bs <= 1024 && bs % WAVE == 0) ? bs : 256;
+    }
+    const size_t lane_waves = lane ? (cap + WAVE - 1) / WAVE : 0;
+    const size_t o_cols = carve(lane_waves * 2 * (size_t)LANE_MAX_WIDTH * WAVE * sizeof(double));
     p->ws_bytes = off;
     if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) return bail(HWT_ENOMEM, "hipMalloc workspace failed");
     char *base = (char *)p->ws;
@@ -583,10 +622,13 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     p->offs = (int32_t *)(base + o_offs);
     p->tally = (int32_t *)(base + o_tally);
     p->scratch_wide = (double *)(base + o_scw);
+    p->cols = lane ? (double *)(base + o_cols) : nullptr;
     if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
-        hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess)
+        hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
+        hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||
+        hipEventCreate(&p->evn0) != hipSuccess || hipEventCreate(&p->evn1) != hipSuccess)
         return bail(HWT_EHIP, "hipEventCreate failed");
     *out = p;
     (void)rc;
@@ -603,6 +645,8 @@ void hwt_plan_destroy(hwt_plan *p)
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->evk0) (void)hipEventDestroy(p->evk0);
     if (p->evk1) (void)hipEventDestroy(p->evk1);
+    for (hipEvent_t e : {p->evw0, p->evw1, p->evn0, p->evn1})
+        if (e) (void)hipEventDestroy(e);
     if (p->fork) (void)hipEventDestroy(p->fork);
     if (p->join) (void)hipEventDestroy(p->join);
     if (p->side) (void)hipStreamDestroy(p->side);
@@ -671,14 +715,21 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, p->ev0, p->ev1) == hipSuccess) stats->total_ms = ms;
         stats->kernel_ms = stats->total_ms;
-        if (p->timed_kernel && hipEventElapsedTime(&ms, p->evk0, p->evk1) == hipSuccess) stats->kernel_ms = ms;
+        if (p->timed_kernel) {
+            if (hipEventElapsedTime(&ms, p->evk0, p->evk1) == hipSuccess) stats->kernel_ms = ms;
+            if (hipEventElapsedTime(&ms, p->evw0, p->evw1) == hipSuccess) stats->wide_ms = ms;
+            if (hipEventElapsedTime(&ms, p->evn0, p->evn1) == hipSuccess) stats->narrow_ms = ms;
+        }
         stats->device_memory = (int64_t)p->ws_bytes;
-        int64_t w, sw;
+        int64_t w, sw, ww;
         std::memcpy(&w, t + T_WORK, sizeof(w));
         std::memcpy(&sw, t + T_SEQWORK, sizeof(sw));
+        std::memcpy(&ww, t + T_WORKW, sizeof(ww));
         stats->node_updates = w;
+        stats->wide_node_updates = ww;
         stats->seq_node_updates = sw;
         stats->tasks = t[T_NTASK];
+        stats->wide_tasks = t[T_NLARGE];
         stats->unsupported = t[T_UNSUP];
     }
     if (t[T_INVALID] > 0) return fail(HWT_EINVAL, std::to_string(t[T_INVALID]) + " option(s) with invalid parameters (price NaN)");

@@ -101,6 +101,7 @@ class DeviceArray:
 
     def fill_zero(self):
         _ck(hip().hipMemset(self.ptr, 0, self.nbytes))
+        synchronize()  # the pricing plans run on their own non-blocking streams
 
     def free(self):
         if self.ptr:

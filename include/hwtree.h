@@ -97,14 +97,20 @@ typedef struct {
 #define HWT_FLAG_NO_GRAPH 1
 
 typedef struct {
-    double  kernel_ms;        /* pricing kernel alone (HWT_FLAG_NO_GRAPH), else = total_ms */
+    double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),
+                                 else = total_ms */
     double  total_ms;         /* whole plan execution: constants, ordering, pricing;
                                  excludes H2D/D2H copies.  HIP events. */
+    double  wide_ms;          /* wide-tree pricing kernel alone (HWT_FLAG_NO_GRAPH), else 0 */
+    double  narrow_ms;        /* narrow-tree pricing kernel alone (HWT_FLAG_NO_GRAPH), else 0 */
     int64_t device_memory;    /* bytes of device memory held by the plan */
     int64_t node_updates;     /* stencil node updates performed (active tree region) */
+    int64_t wide_node_updates;/* ... of which by the wide-tree kernel */
     int64_t seq_node_updates; /* node updates the reference's seq would perform (both passes) */
     int32_t tasks;            /* work items scheduled */
+    int32_t wide_tasks;       /* ... of which wide trees */
     int32_t unsupported;      /* options that could not be priced (output NaN) */
+    int32_t reserved;
 } hwt_stats;
 
 /* Fills *cfg with defaults (auto form, FP64, tuned sizes, work-descending order). */

@@ -1,0 +1,57 @@
+"""Host-side checks of the C ABI library (no GPU needed, no compute calls):
+the library loads and exports every function include/hwtree.h declares, and
+the host-only entry points behave."""
+import ctypes as C
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_functions():
+    text = (ROOT / "include" / "hwtree.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hwt_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_api():
+    names = _declared_functions()
+    for must in ("hwt_price_f64", "hwt_plan_create", "hwt_plan_execute", "hwt_plan_sync",
+                 "hwt_plan_destroy", "hwt_alphas_f64", "hwt_strerror"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = pkg.capi.lib()
+    missing = [n for n in _declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    assert set(pkg.capi.EXPORTS) == set(_declared_functions())
+
+
+def test_host_only_entry_points(pkg):
+    lib = pkg.capi.lib()
+    assert lib.hwt_abi_version() == 1
+    assert lib.hwt_max_width(pkg.capi.FORM_WAVE) == 1344
+    assert lib.hwt_strerror(0) == b"success"
+    assert lib.hwt_strerror(-1) == b"invalid argument"
+    assert lib.hwt_strerror(-2).startswith(b"unsupported")
+    cfg = pkg.capi.config()
+    assert cfg.form == 0 and cfg.precision == 0 and cfg.sort == ord("L") and cfg.device == -1
+
+
+def test_struct_layouts_match_header(pkg):
+    # 8-byte aligned pointers after an int32 count
+    assert C.sizeof(pkg.capi.OptionsC) == 8 + 8 * 8
+    assert C.sizeof(pkg.capi.YieldC) == 24
+    assert C.sizeof(pkg.capi.ConfigC) == 8 * 4
+    assert C.sizeof(pkg.capi.StatsC) == 8 * 8 + 4 * 4
+
+
+def test_null_arguments_rejected_without_gpu(pkg):
+    lib = pkg.capi.lib()
+    assert lib.hwt_price_f64(None, None, None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_plan_create(None, 10, 10, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_plan_sync(None, None) == pkg.capi.HWT_EINVAL
+    lib.hwt_plan_destroy(None)  # no-op
+    assert b"hwt_" in lib.hwt_last_error()

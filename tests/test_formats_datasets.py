@@ -1,0 +1,81 @@
+"""Text formats (common/Arrays.hpp) and the seeded workloads."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def test_read_options_fields(pkg):
+    o = pkg.formats.read_options(GOLDEN / "book.in")
+    assert o.N == 5
+    assert list(o.term_steps) == [1, 2, 3, 4, 5]
+    assert o.term_unit.dtype == np.uint16 and o.type.dtype == np.int8
+    assert np.all(o.maturity == 9.0) and np.all(o.length == 3.0) and np.all(o.type == pkg.PUT)
+
+
+def test_results_round_trip_byte_identical(pkg, tmp_path):
+    # the reference writes std::fixed with max_digits10 (17) decimals
+    line = (GOLDEN / "options-1000.out").read_text().strip()
+    vals = pkg.formats.read_results(GOLDEN / "options-1000.out")
+    assert pkg.formats.format_array(vals) == line
+
+
+def test_options_round_trip(pkg, tmp_path):
+    o = pkg.formats.read_options(GOLDEN / "1_RAND_1000.in")
+    pkg.formats.write_options(tmp_path / "x.in", o)
+    o2 = pkg.formats.read_options(tmp_path / "x.in")
+    for name, _ in pkg.formats.OPTION_FIELDS:
+        np.testing.assert_array_equal(getattr(o, name), getattr(o2, name))
+
+
+def test_bad_type_rejected(pkg):
+    with pytest.raises(ValueError, match="OptionType"):
+        pkg.Options(strike=[1.0], maturity=[1.0], length=[1.0], term_unit=[365], term_steps=[12],
+                    a=[0.1], sigma=[0.01], type=[2])
+
+
+def test_missing_file(pkg, tmp_path):
+    with pytest.raises(FileNotFoundError):
+        pkg.formats.read_options(tmp_path / "nope.in")
+
+
+def test_synthetic_yield(pkg):
+    y = pkg.datasets.synthetic_yield()
+    assert y.N == 120
+    assert y.rates[0] == pytest.approx(0.01) and y.rates[-1] == pytest.approx(0.04)
+    assert y.days[0] == 30 and y.days[-1] == 3650
+    assert np.all(np.diff(y.days) > 0)
+
+
+def test_uniform_1k_shape(pkg, oracle):
+    o = pkg.datasets.uniform_1k()
+    assert o.N == 1000
+    c = oracle.constants(o, 0)
+    assert (c.width, c.n, int(c.t / c.dt)) == (47, 120, 60)
+    assert 60 <= o.strike.min() and o.strike.max() <= 100
+
+
+def test_skewed_60k_mix(pkg, oracle):
+    o = pkg.datasets.skewed_60k()
+    assert o.N == 60000
+    large = o.term_steps == 36
+    assert large.sum() == 6000
+    assert set(np.unique(o.type)) == {0, 1}
+    assert np.all(o.length < o.maturity) and np.all(o.length > 0.19 * o.maturity)
+    # tree shapes: small widths <= 47, large widths 267..1327, heights up to 1080
+    idx = np.r_[np.nonzero(~large)[0][:200], np.nonzero(large)[0][:200]]
+    w = np.array([oracle.constants(o, i).width for i in idx])
+    n = np.array([oracle.constants(o, i).n for i in idx])
+    assert w[:200].max() <= 47 and w[200:].min() >= 267 and w[200:].max() <= 1327
+    assert n.max() <= 1080
+    # seeded: same seed, same batch
+    o2 = pkg.datasets.skewed_60k()
+    np.testing.assert_array_equal(o.a, o2.a)
+
+
+def test_box_mix_outside_fraction(pkg):
+    o = pkg.datasets.box_mix(seed=3, n=20000)
+    outside = (o.a > 0.2) | (o.length > 30.0)
+    assert abs(outside.mean() - 0.05) < 0.002
+    inside = ~outside
+    assert o.a[inside].max() <= 0.2 and o.length[inside].max() <= 30.0

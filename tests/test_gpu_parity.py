@@ -1,0 +1,221 @@
+"""GPU parity: the HIP pricer through its C ABI against the CPU oracle.
+
+Bar (FP64): |gpu - oracle| <= 1e-10 * max(|oracle|, 1), i.e. 1e-10 relative
+with an absolute floor of 1e-10 per 100 notional for prices below 1.
+Full-size batches are checked through size-independent properties
+(put-call parity on the tree, the zero-coupon-bond identity, determinism).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, GOLDEN_SETS, parity_tol
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_parity(got, ref, what=""):
+    err = np.abs(got - ref)
+    bad = np.nonzero(~(err <= parity_tol(ref)))[0]
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first {bad[:5]}: got {got[bad[:5]]} want {ref[bad[:5]]}"
+
+
+@pytest.mark.parametrize("form", ["auto", "lane"])
+@pytest.mark.parametrize("name", GOLDEN_SETS)
+def test_golden_sets(pkg, oracle, golden_yield, name, form):
+    opts = pkg.formats.read_options(GOLDEN / f"{name}.in")
+    got = pkg.capi.price(opts, golden_yield, pkg.capi.config(form=FORMS[form]))
+    assert_parity(got, oracle.price(opts, golden_yield), name)
+    # and against the reference's own golden output (17 decimals, other libm)
+    gold = pkg.formats.read_results(GOLDEN / f"{name}.out")
+    np.testing.assert_allclose(got, gold, rtol=1e-10, atol=1e-11)
+
+
+FORMS = {"auto": 0, "lane": 1, "wave": 2}
+
+
+@pytest.mark.parametrize("form", ["auto", "wave", "lane"])
+@pytest.mark.parametrize("sort", ["L", "-", "w", "W", "h", "H"])
+def test_book_options_all_runners(pkg, oracle, golden_yield, form, sort):
+    """The reference's only GPU test (test/TestCuda.cu:16-99): Hull's book
+    option priced with steps/year 1..100, every runner against seq."""
+    opts = pkg.datasets.book(list(range(1, 101)))
+    f = FORMS[form]
+    got = pkg.capi.price(opts, golden_yield, pkg.capi.config(form=f, sort=sort))
+    assert_parity(got, oracle.price(opts, golden_yield))
+
+
+def test_uniform_1k(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.uniform_1k()
+    assert_parity(pkg.capi.price(opts, y), oracle.price(opts, y))
+
+
+def test_skewed_60k_stratified(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed_60k()
+    got = pkg.capi.price(opts, y)
+    sub = np.arange(0, opts.N, 40)
+    assert_parity(got[sub], oracle.price(opts.take(sub), y), "skewed_60k every 40th")
+    assert np.all(np.isfinite(got)) and np.all(got >= 0)
+    lane = pkg.capi.price(opts, y, pkg.capi.config(form=pkg.capi.FORM_LANE))
+    assert_parity(lane, got, "lane form vs wave form, full batch")
+
+
+def _yield_at(t, y, term_unit=365):
+    d = math.floor(t * term_unit + 0.5)  # C round(): half away from zero (t >= 0)
+    days, rates = y.days, y.rates
+    if d <= days[0]:
+        return rates[0]
+    if d > days[-1]:
+        return rates[-1]
+    i = int(np.searchsorted(days, d))
+    return rates[i - 1] + (d - days[i - 1]) / (days[i] - days[i - 1]) * (rates[i] - rates[i - 1])
+
+
+def test_put_call_parity_full_size(pkg):
+    """C - P = 100 P(n dt) - X P(e dt) holds exactly on the calibrated tree,
+    for every option of the full 60k batch."""
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed_60k()
+    put = opts.take(np.arange(opts.N))
+    put.type[:] = pkg.PUT
+    call = opts.take(np.arange(opts.N))
+    call.type[:] = pkg.CALL
+    p, c = pkg.capi.price(put, y), pkg.capi.price(call, y)
+    dt = 1.0 / opts.term_steps.astype(np.float64)
+    n = (opts.term_steps.astype(np.int64) * opts.maturity).astype(np.int64)
+    e = (opts.length / dt).astype(np.int64)
+    rhs = np.array([100 * math.exp(-_yield_at(n[i] * dt[i], y) * n[i] * dt[i])
+                    - opts.strike[i] * math.exp(-_yield_at(e[i] * dt[i], y) * e[i] * dt[i])
+                    for i in range(opts.N)])
+    np.testing.assert_allclose(c - p, rhs, rtol=0, atol=1e-9)
+
+
+def _one(pkg, **kw):
+    base = dict(strike=[80.0], maturity=[9.0], length=[3.0], term_unit=[365], term_steps=[12],
+                a=[0.1], sigma=[0.01], type=[pkg.PUT])
+    base.update({k: [v] for k, v in kw.items()})
+    return pkg.Options(**base)
+
+
+@pytest.mark.parametrize("form", ["auto", "lane"])
+@pytest.mark.parametrize("kw", [
+    dict(a=3.0, term_steps=1, maturity=6.0, length=2.0, sigma=0.03),     # jmax 1
+    dict(a=3.0, term_steps=1, maturity=6.0, length=2.0, sigma=0.03, type=0),
+    dict(a=0.15, term_steps=1, maturity=8.0, length=3.0, sigma=0.03),    # jmax 2
+    dict(a=0.08, term_steps=1, maturity=9.0, length=4.0, sigma=0.03),    # jmax 3
+    dict(length=0.04),                                                   # exercise at step 0
+    dict(length=8.95),                                                   # exercise at the last step
+    dict(length=9.0),                                                    # exercise == height: bond
+    dict(length=12.0),                                                   # beyond the tree: bond
+    dict(maturity=0.05),                                                 # zero steps
+    dict(strike=0.0, type=0),                                            # deep in the money call
+    dict(strike=1000.0, type=0),                                         # worthless call
+    dict(a=0.0111, term_steps=36, maturity=12.0, length=6.0),            # width near the maximum
+    dict(term_unit=30, term_steps=1),                                    # monthly term units
+])
+def test_edge_cases(pkg, oracle, kw, form):
+    y = pkg.datasets.synthetic_yield()
+    o = _one(pkg, **kw)
+    assert_parity(pkg.capi.price(o, y, pkg.capi.config(form=FORMS[form])), oracle.price(o, y), str(kw))
+
+
+def test_widest_supported_tree(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    # a such that jmax = 671, width 1343 (and 672 -> 1345, beyond the limit)
+    dt = 1 / 36
+    a_ok = -math.log(1 - 0.184 / 670.5) / dt
+    o = _one(pkg, a=a_ok, term_steps=36, maturity=2.0, length=1.0)
+    assert oracle.constants(o, 0).width == 1343
+    assert_parity(pkg.capi.price(o, y), oracle.price(o, y))
+
+
+def test_too_wide_is_unsupported_and_others_priced(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    dt = 1 / 36
+    a_bad = -math.log(1 - 0.184 / 680.5) / dt
+    o = pkg.Options.concat([_one(pkg), _one(pkg, a=a_bad, term_steps=36, maturity=2.0, length=1.0)])
+    assert oracle.constants(o, 1).width > 1344
+    rc, got = pkg.capi.price_status(o, y)
+    assert rc == pkg.capi.HWT_EUNSUPPORTED
+    assert np.isnan(got[1])
+    assert_parity(got[:1], oracle.price(o.take([0]), y))
+
+
+def test_invalid_inputs(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    good = _one(pkg)
+    for kw in (dict(a=0.0), dict(a=-0.1), dict(term_steps=0), dict(term_unit=0),
+               dict(maturity=-1.0), dict(sigma=-0.01)):
+        o = pkg.Options.concat([good, _one(pkg, **kw)])
+        rc, got = pkg.capi.price_status(o, y)
+        assert rc == pkg.capi.HWT_EINVAL, kw
+        assert np.isnan(got[1])
+        assert_parity(got[:1], oracle.price(good, y))
+
+
+def test_empty_batch(pkg):
+    y = pkg.datasets.synthetic_yield()
+    o = pkg.Options(strike=[], maturity=[], length=[], term_unit=[], term_steps=[], a=[], sigma=[], type=[])
+    assert pkg.capi.price(o, y).shape == (0,)
+
+
+def test_alphas_match_oracle(pkg, oracle, golden_yield):
+    opts = pkg.Options.concat([pkg.datasets.book([1, 5, 12, 36]),
+                               pkg.formats.read_options(GOLDEN / "1_RAND_1000.in").take(np.arange(0, 1000, 97))])
+    got = pkg.capi.alphas(opts, golden_yield)
+    for i in range(opts.N):
+        _, ref = oracle.price_single(opts, i, golden_yield, with_alphas=True)
+        assert got[i].shape == ref.shape
+        np.testing.assert_allclose(got[i], ref, rtol=1e-9, atol=1e-12)
+
+
+def test_deterministic_and_order_independent(pkg):
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=9, n=3000)
+    a = pkg.capi.price(opts, y)
+    b = pkg.capi.price(opts, y)
+    assert np.array_equal(a, b)
+    for sort in "-wWhH":
+        assert np.array_equal(a, pkg.capi.price(opts, y, pkg.capi.config(sort=sort)))
+    perm = np.random.default_rng(1).permutation(opts.N)
+    c = pkg.capi.price(opts.take(perm), y)
+    assert np.array_equal(c, a[perm])
+
+
+def test_plan_graph_replays(pkg, oracle):
+    """A plan captured into a HIP graph and replayed: every replay, with other
+    device work in between, equals the first and the oracle."""
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=4, n=2000)
+    dev = pkg.capi.to_device(opts, y)
+    plan = pkg.capi.Plan(opts.N, y.N)
+    plan.execute(dev)
+    plan.sync()
+    first = dev["out"].to_host()
+    assert_parity(first, oracle.price(opts, y))
+    other = pkg.capi.to_device(pkg.datasets.uniform_1k(), y)
+    plan2 = pkg.capi.Plan(1000, y.N, pkg.capi.config(flags=pkg.capi.FLAG_NO_GRAPH))
+    for _ in range(4):
+        dev["out"].fill_zero()
+        plan2.execute(other)
+        plan2.sync()
+        plan.execute(dev)
+        st = plan.sync()
+        assert np.array_equal(dev["out"].to_host(), first)
+        assert st["tasks"] == opts.N and st["unsupported"] == 0
+    plan.close()
+    plan2.close()
+
+
+def test_plan_capacity_enforced(pkg):
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.uniform_1k()
+    dev = pkg.capi.to_device(opts, y)
+    plan = pkg.capi.Plan(10, y.N)
+    assert plan.execute_status(dev) == pkg.capi.HWT_ENOMEM
+    assert plan.execute_status(dev, n=10) == pkg.capi.HWT_OK
+    plan.sync()
+    plan.close()
