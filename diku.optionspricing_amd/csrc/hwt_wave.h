@@ -161,50 +161,73 @@ __device__ __forceinline__ void load_edge_table(const double *src, int lane, lds
 }
 
 // ------------------------------------------------------------- stencil
-// One step from src into dst (distinct register arrays).
-template <int K>
-__device__ __forceinline__ void stencil_step(const double (&src)[K], double (&dst)[K], const double (&cl)[K],
-                                             const double (&cm)[K], const double (&cr)[K])
+// Propagation type T: double (FP64 mode) or float (mixed mode, where the two
+// tree passes run in FP32 and every discount factor and sum stays FP64).
+template <class T> struct Lds;
+template <> struct Lds<double> { typedef __attribute__((address_space(3))) double type; };
+template <> struct Lds<float> { typedef __attribute__((address_space(3))) float type; };
+
+__device__ __forceinline__ float dpp_from_lower_lane(float x)
 {
-    const double left = dpp_from_lower_lane(src[K - 1]);
-    const double right = dpp_from_upper_lane(src[0]);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_from_upper_lane(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
+}
+
+// Largest finite value of the propagation type: a payoff that overflows is
+// clamped to it, as the reference does in either precision
+// (common/Domain.hpp:163-168).
+template <class T> struct RealMax;
+template <> struct RealMax<double> { static constexpr double value = DBL_MAX; };
+template <> struct RealMax<float> { static constexpr double value = FLT_MAX; };
+
+// One step from src into dst (distinct register arrays).
+template <class T, int K>
+__device__ __forceinline__ void stencil_step(const T (&src)[K], T (&dst)[K], const T (&cl)[K], const T (&cm)[K],
+                                             const T (&cr)[K])
+{
+    const T left = dpp_from_lower_lane(src[K - 1]);
+    const T right = dpp_from_upper_lane(src[0]);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-        const double pv = (s > 0) ? src[s - 1] : left;
-        const double nx = (s + 1 < K) ? src[s + 1] : right;
-        double v = cm[s] * src[s];
+        const T pv = (s > 0) ? src[s - 1] : left;
+        const T nx = (s + 1 < K) ? src[s + 1] : right;
+        T v = cm[s] * src[s];
         v = fma(cl[s], pv, v);
-        dst[s] = fma(cr[s], nx, v);
+        v = fma(cr[s], nx, v);
+        dst[s] = v;
     }
 }
 
 // nsteps steps in place; two steps per iteration ping-pong between z and a
 // second register array so no value has to be copied.
-template <int K>
-__device__ __forceinline__ void stencil_steps(double (&z)[K], const double (&cl)[K], const double (&cm)[K],
-                                              const double (&cr)[K], int nsteps)
+template <class T, int K>
+__device__ __forceinline__ void stencil_steps(T (&z)[K], const T (&cl)[K], const T (&cm)[K], const T (&cr)[K],
+                                              int nsteps)
 {
-    double t[K];
+    T t[K];
     for (int it = 0; it + 1 < nsteps; it += 2) {
-        stencil_step<K>(z, t, cl, cm, cr);
-        stencil_step<K>(t, z, cl, cm, cr);
+        stencil_step<T, K>(z, t, cl, cm, cr);
+        stencil_step<T, K>(t, z, cl, cm, cr);
     }
     if (nsteps & 1) {
-        stencil_step<K>(z, t, cl, cm, cr);
+        stencil_step<T, K>(z, t, cl, cm, cr);
 #pragma unroll
         for (int s = 0; s < K; ++s) z[s] = t[s];
     }
 }
 
 // Coefficients of this lane's K nodes for the forward operator G (FWD) or the
-// backward operator G^T.  Interior nodes use the closed forms with the node
-// discounts E_k = e^{-(k-jmax) dr dt} advanced by one multiplication per node;
-// padding beyond the top edge is zero; the (at most six) edge nodes inside the
-// window are then overwritten from the table, with a wave-uniform switch on
-// their slot so every register index stays static.
-template <int K, bool FWD>
+// backward operator G^T, computed in FP64 and stored as T.  Interior nodes use
+// the closed forms with the node discounts E_k = e^{-(k-jmax) dr dt} advanced
+// by one multiplication per node; padding beyond the top edge is zero; the (at
+// most six) edge nodes inside the window are then overwritten from the table,
+// with a wave-uniform switch on their slot so every register index stays static.
+template <class T, int K, bool FWD>
 __device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int lane, const lds_f64 *etab,
-                                             double (&cl)[K], double (&cm)[K], double (&cr)[K])
+                                             T (&cl)[K], T (&cm)[K], T (&cr)[K])
 {
     const int k0 = base + lane * K;
     const double q = exp(-t.dr * t.dt);
@@ -217,23 +240,23 @@ __device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int l
             const int k = k0 - 1 + m;
             const int j = k - t.jmax;
             const double Ek = (k >= 0 && k < t.w) ? E : 0.0;
-            if (m < K) cl[m < K ? m : 0] = pu_std(j, t.M) * Ek;
-            if (m >= 1 && m <= K) cm[(m >= 1 && m <= K) ? m - 1 : 0] = pm_std(j, t.M) * Ek;
-            if (m >= 2) cr[m >= 2 ? m - 2 : 0] = pd_std(j, t.M) * Ek;
+            if (m < K) cl[m < K ? m : 0] = (T)(pu_std(j, t.M) * Ek);
+            if (m >= 1 && m <= K) cm[(m >= 1 && m <= K) ? m - 1 : 0] = (T)(pm_std(j, t.M) * Ek);
+            if (m >= 2) cr[m >= 2 ? m - 2 : 0] = (T)(pd_std(j, t.M) * Ek);
             E *= q;
         }
 #pragma unroll
         for (int s = 0; s < K; ++s)
-            if (k0 + s >= t.w) cl[s] = cm[s] = cr[s] = 0.0;
+            if (k0 + s >= t.w) cl[s] = cm[s] = cr[s] = (T)0;
     } else {
         double E = node_disc(k0, t.jmax, t.dr, t.dt);
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             const int j = k0 + s - t.jmax;
             const double Ek = (k0 + s < t.w) ? E : 0.0;
-            cl[s] = pd_std(j, t.M) * Ek;
-            cm[s] = pm_std(j, t.M) * Ek;
-            cr[s] = pu_std(j, t.M) * Ek;
+            cl[s] = (T)(pd_std(j, t.M) * Ek);
+            cm[s] = (T)(pm_std(j, t.M) * Ek);
+            cr[s] = (T)(pu_std(j, t.M) * Ek);
             E *= q;
         }
     }
@@ -244,7 +267,7 @@ __device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int l
         const int off = k - base;
         if (k >= 0 && k < t.w && off >= 0 && off < WAVE * K) {
             const int L = off / K, S = off - L * K;  // wave-uniform
-            const double a = tab[3 * e], b = tab[3 * e + 1], c = tab[3 * e + 2];
+            const T a = (T)tab[3 * e], b = (T)tab[3 * e + 1], c = (T)tab[3 * e + 2];
             if (lane == L) {
                 switch (S) {
 #define HWT_SLOT(SS)                                       \
@@ -267,8 +290,8 @@ __device__ __forceinline__ void window_coefs(const TreeShape &t, int base, int l
 }
 
 // Initial values: forward delta at the root node; backward T^{-T} * 100.
-template <int K, bool FWD>
-__device__ __forceinline__ void init_window(double (&z)[K], const TreeShape &t, int base, int lane)
+template <class T, int K, bool FWD>
+__device__ __forceinline__ void init_window(T (&z)[K], const TreeShape &t, int base, int lane)
 {
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -281,32 +304,32 @@ __device__ __forceinline__ void init_window(double (&z)[K], const TreeShape &t, 
             if (k == 0) v -= t.rho * 100.0;
             if (k == t.w - 1) v -= t.rhop * 100.0;
         }
-        z[s] = v;
+        z[s] = (T)v;
     }
 }
 
 // One phase: bring the window in (initial values, or the previous window's
 // nodes [in_lo, in_hi) from LDS, zeros elsewhere), run nsteps stencil steps,
 // and leave the window's tree nodes in LDS, or in `out` when given.
-template <int K, bool FWD>
+template <class T, int K, bool FWD>
 __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nsteps, int in_lo, int in_hi, int lane,
-                                         lds_f64 *lds, const lds_f64 *etab, double *out)
+                                         typename Lds<T>::type *lds, const lds_f64 *etab, T *out)
 {
-    double z[K];
+    T z[K];
     if (in_lo >= in_hi) {
-        init_window<K, FWD>(z, t, base, lane);
+        init_window<T, K, FWD>(z, t, base, lane);
     } else {
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             const int k = base + lane * K + s;
-            z[s] = (k >= in_lo && k < in_hi) ? lds[k] : 0.0;
+            z[s] = (k >= in_lo && k < in_hi) ? lds[k] : (T)0;
         }
     }
     __syncthreads();
     if (nsteps > 0) {
-        double cl[K], cm[K], cr[K];
-        window_coefs<K, FWD>(t, base, lane, etab, cl, cm, cr);
-        stencil_steps<K>(z, cl, cm, cr, nsteps);
+        T cl[K], cm[K], cr[K];
+        window_coefs<T, K, FWD>(t, base, lane, etab, cl, cm, cr);
+        stencil_steps<T, K>(z, cl, cm, cr, nsteps);
     }
     if (out) {
 #pragma unroll
@@ -327,15 +350,15 @@ __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nstep
 #define HWT_KCASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) \
     X(14) X(15) X(16) X(17) X(18) X(19) X(20) X(21)
 
-template <int KMAX, bool FWD>
+template <class T, int KMAX, bool FWD>
 __device__ __forceinline__ void dispatch_phase(int K, const TreeShape &t, int base, int nsteps, int in_lo, int in_hi,
-                                               int lane, lds_f64 *lds, const lds_f64 *etab, double *out)
+                                               int lane, typename Lds<T>::type *lds, const lds_f64 *etab, T *out)
 {
     switch (K) {
-#define HWT_C(KK)                                                                                     \
-    case KK:                                                                                          \
-        if (KK <= KMAX)                                                                               \
-            phase_fn<(KK <= KMAX ? KK : 1), FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab, out); \
+#define HWT_C(KK)                                                                                        \
+    case KK:                                                                                             \
+        if (KK <= KMAX)                                                                                  \
+            phase_fn<T, (KK <= KMAX ? KK : 1), FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab, out); \
         break;
         HWT_KCASES(HWT_C)
 #undef HWT_C
@@ -358,9 +381,9 @@ __device__ __forceinline__ Window window_for(int D, const TreeShape &t)
 
 // Forward pass: state 0 (delta at the root) -> state e.  The state ends up in
 // out[lo, hi) (global scratch); tree nodes outside that range are zero.
-template <int KMAX>
-__device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, lds_f64 *lds, const lds_f64 *etab,
-                                             double *out, int &lo, int &hi)
+template <class T, int KMAX>
+__device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, typename Lds<T>::type *lds,
+                                             const lds_f64 *etab, T *out, int &lo, int &hi)
 {
     Window win = window_for(0, t);
     int in_lo = 0, in_hi = 0;  // empty: initial values
@@ -369,7 +392,8 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
         const bool full = 32 * win.K > t.jmax;  // growth windows satisfy 32K <= jmax
         const int iend = full ? e : min(e, 32 * win.K - 1);
         const bool last = iend >= e;
-        dispatch_phase<KMAX, true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab, last ? out : nullptr);
+        dispatch_phase<T, KMAX, true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab,
+                                      last ? out : nullptr);
         i = iend;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -383,8 +407,8 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
 // Backward pass: state n (bond face value 100) -> state e, keeping only the
 // nodes that can still reach the exercise nodes |j| <= min(e, jmax).  The
 // state ends up in lds[lo, hi).
-template <int KMAX>
-__device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, lds_f64 *lds,
+template <class T, int KMAX>
+__device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, typename Lds<T>::type *lds,
                                               const lds_f64 *etab, int &lo, int &hi)
 {
     const int ep = min(e, t.jmax);
@@ -396,7 +420,7 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
         const bool full = 32 * win.K > t.jmax;
         const int klo = full ? Kgmax : win.K - 1;
         const int send = max(e, e + 32 * klo - 1 - ep);
-        dispatch_phase<KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, nullptr);
+        dispatch_phase<T, KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, (T *)nullptr);
         s = send;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -405,5 +429,27 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
     lo = in_lo;
     hi = in_hi;
 }
+
+// FP64 accumulators for the duality sums: plain in FP64 mode; compensated
+// (Neumaier) in mixed mode, where the FP32 tree states are summed in FP64.
+struct PlainSum {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double wave_total() const { return wave_sum(s); }
+};
+
+struct KahanSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double t = s + x;
+        c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double wave_total() const { return wave_sum(s) + wave_sum(c); }
+};
+
+template <class T> struct SumOf { typedef PlainSum type; };
+template <> struct SumOf<float> { typedef KahanSum type; };
 
 } // namespace hwt

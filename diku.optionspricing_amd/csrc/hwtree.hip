@@ -176,21 +176,37 @@ __global__ void k_scatter(int32_t n, const int32_t *__restrict__ bucket, const i
     if (b >= 0) order[offs[b] + atomicAdd(&cursor[b], 1)] = i;
 }
 
-template <int KMAX>
-__device__ __forceinline__ double price_option(const OptConst &cv, const double *edge, int lane, lds_f64 *lds,
-                                               lds_f64 *etab, double *fscratch)
+// Payoff of the option on a bond worth B at the exercise step, clamped to the
+// largest finite value of the propagation type (common/Domain.hpp:163-168).
+template <class T>
+__device__ __forceinline__ double payoff(const OptConst &cv, double B)
+{
+    double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
+    v = (v < 0.0) ? 0.0 : v;
+    return (v > RealMax<T>::value) ? RealMax<T>::value : v;
+}
+
+// Price from the two states at the exercise step: y (forward, Arrow-Debreu)
+// and b (backward, bond value per unit face), both untransformed:
+//   S1 = sum y, S2 = sum y b, scale = 100 S1 Pn / (S2 Pe),
+//   price = Pe / S1 * sum y payoff(scale b).
+// Sums are FP64 (compensated in mixed mode).
+
+template <class T, int KMAX>
+__device__ __forceinline__ double price_option(const OptConst &cv, const double *edge, int lane,
+                                               typename Lds<T>::type *lds, lds_f64 *etab, T *fscratch)
 {
     const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
     const int n = uni(cv.n), e = uni(cv.e);
     load_edge_table(edge, lane, etab);
     int flo, fhi, blo, bhi;
-    forward_pass<KMAX>(t, e, lane, lds, etab, fscratch, flo, fhi);
-    backward_pass<KMAX>(t, n, e, lane, lds, etab, blo, bhi);
+    forward_pass<T, KMAX>(t, e, lane, lds, etab, fscratch, flo, fhi);
+    backward_pass<T, KMAX>(t, n, e, lane, lds, etab, blo, bhi);
 
     const int ep = min(e, t.jmax);
     const int klo = t.jmax - ep, khi = t.jmax + ep;
-    auto fy = [&](int k) { return (k >= flo && k < fhi) ? fscratch[k] : 0.0; };
-    auto fb = [&](int k) { return (k >= blo && k < bhi) ? lds[k] : 0.0; };
+    auto fy = [&](int k) { return (k >= flo && k < fhi) ? (double)fscratch[k] : 0.0; };
+    auto fb = [&](int k) { return (k >= blo && k < bhi) ? (double)lds[k] : 0.0; };
     auto yk = [&](int k) {
         double v = fy(k);
         if (k == 1) v -= t.rho * fy(0);
@@ -203,54 +219,46 @@ __device__ __forceinline__ double price_option(const OptConst &cv, const double 
         if (k == t.w - 1) v += t.rhop * fb(t.w - 2);
         return v;
     };
-    double s1 = 0.0, s2 = 0.0;
+    typename SumOf<T>::type a1, a2, a3;
     for (int k = klo + lane; k <= khi; k += WAVE) {
         const double yv = yk(k);
-        s1 += yv;
-        s2 += yv * bk(k);
+        a1.add(yv);
+        a2.add(yv * bk(k));
     }
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
+    const double s1 = a1.wave_total(), s2 = a2.wave_total();
     const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
-    double s3 = 0.0;
-    for (int k = klo + lane; k <= khi; k += WAVE) {
-        const double B = scale * bk(k);
-        double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
-        v = (v < 0.0) ? 0.0 : v;
-        if (v == INFINITY) v = DBL_MAX;  // common/Domain.hpp:163-168
-        s3 += yk(k) * v;
-    }
-    s3 = wave_sum(s3);
+    for (int k = klo + lane; k <= khi; k += WAVE) a3.add(yk(k) * payoff<T>(cv, scale * bk(k)));
+    const double s3 = a3.wave_total();
     __syncthreads();
     return cv.Pe / s1 * s3;
 }
 
 // Narrow trees (width <= 64*K): the whole tree in one full window, both passes
 // in registers, and the duality sums formed straight from the registers.
-template <int K>
+template <class T, int K>
 __device__ __forceinline__ double price_narrow(const OptConst &cv, const double *edge, int lane, lds_f64 *etab)
 {
     const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
     const int n = uni(cv.n), e = uni(cv.e);
     load_edge_table(edge, lane, etab);
-    double cl[K], cm[K], cr[K], zf[K], zb[K];
-    window_coefs<K, true>(t, 0, lane, etab, cl, cm, cr);
-    init_window<K, true>(zf, t, 0, lane);
-    stencil_steps<K>(zf, cl, cm, cr, e);
-    window_coefs<K, false>(t, 0, lane, etab, cl, cm, cr);
-    init_window<K, false>(zb, t, 0, lane);
-    stencil_steps<K>(zb, cl, cm, cr, n - e);
+    T cl[K], cm[K], cr[K], zf[K], zb[K];
+    window_coefs<T, K, true>(t, 0, lane, etab, cl, cm, cr);
+    init_window<T, K, true>(zf, t, 0, lane);
+    stencil_steps<T, K>(zf, cl, cm, cr, e);
+    window_coefs<T, K, false>(t, 0, lane, etab, cl, cm, cr);
+    init_window<T, K, false>(zb, t, 0, lane);
+    stencil_steps<T, K>(zb, cl, cm, cr, n - e);
 
     // undo the edge similarity: y = T^{-1} zf, b = T^T zb
     const double fl = dpp_from_lower_lane(zf[K - 1]), fr = dpp_from_upper_lane(zf[0]);
     const double bl = dpp_from_lower_lane(zb[K - 1]), br = dpp_from_upper_lane(zb[0]);
     double yv[K], bv[K];
-    double s1 = 0.0, s2 = 0.0;
+    typename SumOf<T>::type a1, a2, a3;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
         const int k = lane * K + s;
-        const double fL = s > 0 ? zf[s - 1] : fl, fR = s + 1 < K ? zf[s + 1] : fr;
-        const double bL = s > 0 ? zb[s - 1] : bl, bR = s + 1 < K ? zb[s + 1] : br;
+        const double fL = s > 0 ? (double)zf[s - 1] : fl, fR = s + 1 < K ? (double)zf[s + 1] : fr;
+        const double bL = s > 0 ? (double)zb[s - 1] : bl, bR = s + 1 < K ? (double)zb[s + 1] : br;
         double y = zf[s], b = zb[s];
         if (k == 1) y -= t.rho * fL;
         if (k == t.w - 2) y -= t.rhop * fR;
@@ -259,22 +267,14 @@ __device__ __forceinline__ double price_narrow(const OptConst &cv, const double 
         if (k >= t.w) y = b = 0.0;
         yv[s] = y;
         bv[s] = b;
-        s1 += y;
-        s2 += y * b;
+        a1.add(y);
+        a2.add(y * b);
     }
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
+    const double s1 = a1.wave_total(), s2 = a2.wave_total();
     const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
-    double s3 = 0.0;
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-        const double B = scale * bv[s];
-        double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
-        v = (v < 0.0) ? 0.0 : v;
-        if (v == INFINITY) v = DBL_MAX;  // common/Domain.hpp:163-168
-        s3 += yv[s] * v;
-    }
-    s3 = wave_sum(s3);
+    for (int s = 0; s < K; ++s) a3.add(yv[s] * payoff<T>(cv, scale * bv[s]));
+    const double s3 = a3.wave_total();
     return cv.Pe / s1 * s3;
 }
 
@@ -285,16 +285,16 @@ __device__ __forceinline__ double price_narrow(const OptConst &cv, const double 
 // claims the next task in one divergent region; every lane then reads the
 // claim back through a convergent broadcast, so all 64 lanes always leave the
 // loop together.
-template <int KMAX, bool NARROW>
+template <class T, int KMAX, bool NARROW>
 __device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                             const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                             int queue, int first, int last, double *__restrict__ out,
                                             double *__restrict__ scratch)
 {
-    __shared__ double lds[NARROW ? 1 : WAVE * KMAX];
+    __shared__ T lds[NARROW ? 1 : WAVE * KMAX];
     __shared__ double etab[EDGE_TAB];
     const int lane = threadIdx.x;
-    double *fscratch = scratch + (size_t)blockIdx.x * (WAVE * KMAX);
+    T *fscratch = reinterpret_cast<T *>(scratch + (size_t)blockIdx.x * (WAVE * KMAX));
     const int stride = uni((int)gridDim.x);
     int t = uni((int)blockIdx.x) + first;
     while (t < last) {
@@ -304,10 +304,10 @@ __device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, co
         double p;
         if (NARROW) {
             const int K = (uni(c.w) + WAVE - 1) / WAVE;
-            p = (K == 1) ? price_narrow<1>(c, eo, lane, (lds_f64 *)etab)
-                         : price_narrow<(KMAX >= 2 ? 2 : 1)>(c, eo, lane, (lds_f64 *)etab);
+            p = (K == 1) ? price_narrow<T, 1>(c, eo, lane, (lds_f64 *)etab)
+                         : price_narrow<T, (KMAX >= 2 ? 2 : 1)>(c, eo, lane, (lds_f64 *)etab);
         } else {
-            p = price_option<KMAX>(c, eo, lane, (lds_f64 *)lds, (lds_f64 *)etab, fscratch);
+            p = price_option<T, KMAX>(c, eo, lane, (typename Lds<T>::type *)lds, (lds_f64 *)etab, fscratch);
         }
         int claim = 0;
         if (lane == 0) {
@@ -318,20 +318,22 @@ __device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, co
     }
 }
 
+template <class T>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void k_price_wide(
     const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
     int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
 {
     const int nlarge = uni(tally[T_NLARGE]);
-    price_tasks<MAXK, false>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
+    price_tasks<T, MAXK, false>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
 }
 
+template <class T>
 __global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                                        const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                                        double *__restrict__ out)
 {
     const int nlarge = uni(tally[T_NLARGE]), ntask = uni(tally[T_NTASK]);
-    price_tasks<KSMALL, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
+    price_tasks<T, KSMALL, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
 }
 
 // Form 1: one lane per option over every scheduled task (tasks ordered by
@@ -358,8 +360,8 @@ __device__ __noinline__ void alphas_full(const OptConst c, const DevYield y, int
 {
     const TreeShape t = uni(TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop});
     double z[K], cl[K], cm[K], cr[K], wE[K];
-    window_coefs<K, true>(t, 0, lane, etab, cl, cm, cr);
-    init_window<K, true>(z, t, 0, lane);
+    window_coefs<double, K, true>(t, 0, lane, etab, cl, cm, cr);
+    init_window<double, K, true>(z, t, 0, lane);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
         const int k = lane * K + s;
@@ -370,7 +372,7 @@ __device__ __noinline__ void alphas_full(const OptConst c, const DevYield y, int
     }
     double Aprev = 0.0;
     for (int i = 0; i <= c.n; ++i) {
-        if (i > 0) stencil_steps<K>(z, cl, cm, cr, 1);
+        if (i > 0) stencil_steps<double, K>(z, cl, cm, cr, 1);
         double part = 0.0;
 #pragma unroll
         for (int s = 0; s < K; ++s) part += wE[s] * z[s];
@@ -475,12 +477,18 @@ struct hwt_plan {
     double *g_out = nullptr;
     hipStream_t g_stream = nullptr;
     hipStream_t last_stream = nullptr;
-    bool timed_kernel = false;
+    bool timed_kernel = false;  // evk0/evk1 were recorded by the last execution
+    bool timed_split = false;   // so were evw0/evw1 and evn0/evn1
 };
 
 static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, double *out, hipStream_t s, bool time_kernel)
 {
     const int n = o.n;
+    // Report only this launch's errors: an error left by an unrelated earlier
+    // runtime call would otherwise surface in the check below.
+    (void)hipGetLastError();
+    p->timed_kernel = time_kernel && n > 0;
+    p->timed_split = p->timed_kernel && p->cfg.form != HWT_FORM_LANE;
     const int sort = p->cfg.sort ? p->cfg.sort : 'L';
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
     k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
@@ -500,10 +508,16 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         HWT_HIP(hipEventRecord(p->fork, s));
         HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
         if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, p->side));
-        k_price_narrow<<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
+        if (p->cfg.precision == HWT_PREC_MIXED)
+            k_price_narrow<float><<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
+        else
+            k_price_narrow<double><<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, p->side));
         if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
-        k_price_wide<<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
+        if (p->cfg.precision == HWT_PREC_MIXED)
+            k_price_wide<float><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
+        else
+            k_price_wide<double><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evw1, s));
         HWT_HIP(hipEventRecord(p->join, p->side));
         HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
@@ -556,9 +570,14 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     if (!p) return fail(HWT_ENOMEM, "hwt_plan_create: host allocation");
     if (cfg) p->cfg = *cfg;
     else hwt_config_init(&p->cfg);
-    if (p->cfg.form < HWT_FORM_AUTO || p->cfg.form > HWT_FORM_WAVE || p->cfg.precision != HWT_PREC_FP64) {
+    if (p->cfg.form < HWT_FORM_AUTO || p->cfg.form > HWT_FORM_WAVE ||
+        (p->cfg.precision != HWT_PREC_FP64 && p->cfg.precision != HWT_PREC_MIXED)) {
         delete p;
-        return fail(HWT_EUNSUPPORTED, "hwt_plan_create: form/precision not available");
+        return fail(HWT_EINVAL, "hwt_plan_create: unknown form or precision");
+    }
+    if (p->cfg.form == HWT_FORM_LANE && p->cfg.precision == HWT_PREC_MIXED) {
+        delete p;
+        return fail(HWT_EUNSUPPORTED, "hwt_plan_create: the lane-per-option form runs in FP64 only");
     }
     p->capacity = capacity;
     p->ycap = yield_capacity;
@@ -578,9 +597,16 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     int cus = 0, per_cu_w = 0, per_cu_n = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus < 1)
         cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_w, k_price_wide, WAVE, 0) != hipSuccess || per_cu_w < 1)
+    const bool mixed = p->cfg.precision == HWT_PREC_MIXED;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu_w, mixed ? (const void *)k_price_wide<float> : (const void *)k_price_wide<double>, WAVE, 0) !=
+            hipSuccess ||
+        per_cu_w < 1)
         per_cu_w = 8;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_n, k_price_narrow, WAVE, 0) != hipSuccess || per_cu_n < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu_n, mixed ? (const void *)k_price_narrow<float> : (const void *)k_price_narrow<double>, WAVE, 0) !=
+            hipSuccess ||
+        per_cu_n < 1)
         per_cu_n = 8;
     p->grid_wide = per_cu_w * cus;
     p->grid_narrow = per_cu_n * cus;
@@ -671,7 +697,6 @@ int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_
     if (p->cfg.flags & HWT_FLAG_NO_GRAPH) {
         const int rc = launch_pipeline(p, o, y, dout, s, true);
         if (rc) return rc;
-        p->timed_kernel = o.n > 0;
     } else {
         const bool same = p->exec && p->g_stream == s && p->g_out == dout &&
                           std::memcmp(&p->g_opts, &o, sizeof(o)) == 0 && std::memcmp(&p->g_yield, &y, sizeof(y)) == 0;
@@ -697,7 +722,7 @@ int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_
             p->g_stream = s;
         }
         HWT_HIP(hipGraphLaunch(p->exec, s));
-        p->timed_kernel = false;
+        p->timed_kernel = p->timed_split = false;
     }
     HWT_HIP(hipEventRecord(p->ev1, s));
     HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -717,6 +742,8 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
         stats->kernel_ms = stats->total_ms;
         if (p->timed_kernel) {
             if (hipEventElapsedTime(&ms, p->evk0, p->evk1) == hipSuccess) stats->kernel_ms = ms;
+        }
+        if (p->timed_split) {
             if (hipEventElapsedTime(&ms, p->evw0, p->evw1) == hipSuccess) stats->wide_ms = ms;
             if (hipEventElapsedTime(&ms, p->evn0, p->evn1) == hipSuccess) stats->narrow_ms = ms;
         }

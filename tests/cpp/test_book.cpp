@@ -93,7 +93,7 @@ int main(int argc, char **argv)
     try {
         std::vector<double> results(options.N);
         cuda::multi::KernelRunNaive k;
-        k.run(options, yield, results, 512);  // 100 steps/yr gives widths above 512
+        k.run(options, yield, results, 256);  // 100 steps/yr gives width 371
     } catch (const std::invalid_argument &) {
         threw = true;
     }
