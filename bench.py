@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: one 64 MiB device-to-device copy (the byte-counter calibration "
+                         "dispatch), then warmup + steps executions of the pipeline without the graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): options are independent, "
@@ -90,6 +93,24 @@ def main():
     y = pkg.datasets.synthetic_yield()
     opts = pkg.datasets.skewed_60k()
     dev = capi.to_device(opts, y)
+
+    if args.profile:
+        n = 64 << 20
+        src, dst = dm.DeviceArray(n, np.uint8, (n,)), dm.DeviceArray(n, np.uint8, (n,))
+        dm._ck(dm.hip().hipMemcpy(dst.ptr, src.ptr, n, dm.D2D))
+        dm.synchronize()
+        plan = capi.Plan(opts.N, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+        wide = []
+        for i in range(args.warmup + args.steps):
+            plan.execute(dev)
+            st = plan.sync()
+            if i >= args.warmup:
+                wide.append(st["wide_ms"])
+            print(f"profile step {i}: wide kernel {st['wide_ms']:.4f} ms", flush=True)
+        plan.close()
+        print(json.dumps({"profile": "skewed_60k", "copy_bytes": n, "wide_ms_mean": float(np.mean(wide)),
+                          "wide_node_updates": st["wide_node_updates"]}), flush=True)
+        return
 
     # whole-job throughput: graph replays on a caller stream, HIP events around K steps
     import ctypes as C

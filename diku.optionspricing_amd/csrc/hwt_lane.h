@@ -32,7 +32,9 @@ __device__ __forceinline__ void lane_sweep(const TreeShape &t, const double *eta
     const double q = exp(-t.dr * t.dt);
     double Em = node_disc(lo - 1, t.jmax, t.dr, t.dt);  // E_{k-1}
     double E0 = Em * q, Ep = E0 * q;
-    double prev = 0.0;  // old value at lo-1: outside the old region or the tree
+    // old value at lo-1: inside the old region when the backward light cone
+    // narrows, zero when it lies outside the old region or the tree
+    double prev = (lo - 1 >= olo && lo - 1 <= ohi && lo >= 1) ? col[lo - 1] : 0.0;
     double cur = (lo >= olo && lo <= ohi) ? col[lo] : 0.0;
     for (int k = lo; k <= hi; ++k) {
         const double nxt = (k + 1 >= olo && k + 1 <= ohi) ? col[k + 1] : 0.0;

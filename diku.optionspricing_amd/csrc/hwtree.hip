@@ -91,58 +91,84 @@ __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
     }
 }
 
-__global__ void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth, OptConst *__restrict__ cst,
-                        double *__restrict__ edge, int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
-                        int32_t *__restrict__ tally, double *__restrict__ out)
+// One thread per option; launched with SETUP_BLOCK threads per block.  The
+// histogram and the work tallies are aggregated per block in LDS first, so the
+// global atomics number a few per block instead of one per option (single
+// counters hit by every option serialise in L2).
+constexpr int SETUP_BLOCK = 256;
+
+__global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
+                                                       OptConst *__restrict__ cst, double *__restrict__ edge,
+                                                       int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
+                                                       int32_t *__restrict__ tally, double *__restrict__ out)
 {
-    const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i >= o.n) return;
-    OptConst c;
-    const int32_t st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i],
-                                        o.term_steps[i], o.a[i], o.sigma[i], o.type[i], maxWidth, c);
-    c.status = st;
-    int b = -1;
-    if (st == ST_INVALID) {
-        out[i] = NAN;
-        atomicAdd(&tally[T_INVALID], 1);
-    } else if (st == ST_UNSUPPORTED) {
-        out[i] = NAN;
-        atomicAdd(&tally[T_UNSUP], 1);
-    } else if (c.n == 0) {
-        out[i] = 100.0;  // no steps: the face value (seq/Seq.hpp:119,171)
-    } else {
-        const int tu = o.term_unit[i];
-        const double tn = c.n * c.dt;
-        c.Pn = exp(-yield_at(tn, tu, y.rate, y.day, y.n) * tn);
-        if (c.e < 0) {
-            out[i] = 100.0 * c.Pn;  // exercise outside the tree: the zero-coupon bond
+    __shared__ int32_t lh[NBUCKET];
+    __shared__ unsigned long long lsum[3];  // work, seq work, wide work
+    __shared__ int32_t lbad[2];             // invalid, unsupported
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lh[k] = 0;
+    if (tid < 3) lsum[tid] = 0ull;
+    if (tid < 2) lbad[tid] = 0;
+    __syncthreads();
+    const int i = tid + blockIdx.x * SETUP_BLOCK;
+    if (i < o.n) {
+        OptConst c;
+        const int32_t st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i],
+                                            o.term_steps[i], o.a[i], o.sigma[i], o.type[i], maxWidth, c);
+        c.status = st;
+        int b = -1;
+        if (st == ST_INVALID) {
+            out[i] = NAN;
+            atomicAdd(&lbad[0], 1);
+        } else if (st == ST_UNSUPPORTED) {
+            out[i] = NAN;
+            atomicAdd(&lbad[1], 1);
+        } else if (c.n == 0) {
+            out[i] = 100.0;  // no steps: the face value (seq/Seq.hpp:119,171)
         } else {
-            const double te = c.e * c.dt;
-            c.Pe = exp(-yield_at(te, tu, y.rate, y.day, y.n) * te);
-            TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
-            edge_factors(t);
-            c.rho = t.rho;
-            c.rhop = t.rhop;
-            edge_table(t, edge + (size_t)i * EDGE_TAB);
-            c.work = dual_node_updates(c.n, c.e, c.jmax);
-            const int64_t seqw = seq_node_updates(c.n, c.jmax);
-            atomicAdd((unsigned long long *)&tally[T_WORK], (unsigned long long)c.work);
-            atomicAdd((unsigned long long *)&tally[T_SEQWORK], (unsigned long long)seqw);
-            int64_t key;
-            switch (sort) {
-            case 'W': case 'w': key = c.w; break;
-            case 'H': case 'h': key = c.n; break;
-            case '-': key = 0; break;
-            default: key = c.work; break;
+            const int tu = o.term_unit[i];
+            const double tn = c.n * c.dt;
+            c.Pn = exp(-yield_at(tn, tu, y.rate, y.day, y.n) * tn);
+            if (c.e < 0) {
+                out[i] = 100.0 * c.Pn;  // exercise outside the tree: the zero-coupon bond
+            } else {
+                const double te = c.e * c.dt;
+                c.Pe = exp(-yield_at(te, tu, y.rate, y.day, y.n) * te);
+                TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
+                edge_factors(t);
+                c.rho = t.rho;
+                c.rhop = t.rhop;
+                edge_table(t, edge + (size_t)i * EDGE_TAB);
+                c.work = dual_node_updates(c.n, c.e, c.jmax);
+                const int64_t seqw = seq_node_updates(c.n, c.jmax);
+                atomicAdd(&lsum[0], (unsigned long long)c.work);
+                atomicAdd(&lsum[1], (unsigned long long)seqw);
+                int64_t key;
+                switch (sort) {
+                case 'W': case 'w': key = c.w; break;
+                case 'H': case 'h': key = c.n; break;
+                case '-': key = 0; break;
+                default: key = c.work; break;
+                }
+                const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
+                b = bucket_of(key, wide);
+                atomicAdd(&lh[b], 1);
+                if (wide) atomicAdd(&lsum[2], (unsigned long long)c.work);
             }
-            const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
-            b = bucket_of(key, wide);
-            atomicAdd(&hist[b], 1);
-            if (wide) atomicAdd((unsigned long long *)&tally[T_WORKW], (unsigned long long)c.work);
         }
+        cst[i] = c;
+        bucket[i] = b;
     }
-    cst[i] = c;
-    bucket[i] = b;
+    __syncthreads();
+    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
+        if (lh[k]) atomicAdd(&hist[k], lh[k]);
+    if (tid == 0) {
+        if (lsum[0]) atomicAdd((unsigned long long *)&tally[T_WORK], lsum[0]);
+        if (lsum[1]) atomicAdd((unsigned long long *)&tally[T_SEQWORK], lsum[1]);
+        if (lsum[2]) atomicAdd((unsigned long long *)&tally[T_WORKW], lsum[2]);
+        if (lbad[0]) atomicAdd(&tally[T_INVALID], lbad[0]);
+        if (lbad[1]) atomicAdd(&tally[T_UNSUP], lbad[1]);
+    }
 }
 
 // Exclusive offsets of the buckets: every wide-tree bucket (128..255) before
@@ -167,13 +193,25 @@ __global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ o
     if (tid == NBUCKET - 1) tally[T_NTASK] = s[tid];
 }
 
-__global__ void k_scatter(int32_t n, const int32_t *__restrict__ bucket, const int32_t *__restrict__ offs,
-                          int32_t *__restrict__ cursor, int32_t *__restrict__ order)
+// Option indices into the task list, bucket by bucket.  Ranks inside a block
+// come from LDS atomics; one global atomic per (block, bucket) reserves the
+// block's slots.  Order inside a bucket is arbitrary (options are independent).
+__global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_t *__restrict__ bucket,
+                                                         const int32_t *__restrict__ offs,
+                                                         int32_t *__restrict__ cursor, int32_t *__restrict__ order)
 {
-    const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i >= n) return;
-    const int b = bucket[i];
-    if (b >= 0) order[offs[b] + atomicAdd(&cursor[b], 1)] = i;
+    __shared__ int32_t lcount[NBUCKET], lbase[NBUCKET];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lcount[k] = 0;
+    __syncthreads();
+    const int i = tid + blockIdx.x * SETUP_BLOCK;
+    const int b = (i < n) ? bucket[i] : -1;
+    const int r = (b >= 0) ? atomicAdd(&lcount[b], 1) : 0;
+    __syncthreads();
+    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
+        if (lcount[k]) lbase[k] = atomicAdd(&cursor[k], lcount[k]);
+    __syncthreads();
+    if (b >= 0) order[offs[b] + lbase[b] + r] = i;
 }
 
 // Payoff of the option on a bond worth B at the exercise step, clamped to the
@@ -279,13 +317,18 @@ __device__ __forceinline__ double price_narrow(const OptConst &cv, const double 
 }
 
 // Persistent pricing kernels: one single-wave workgroup per resident slot;
-// the waves pull tasks [first, last) off the shared list until it is drained.
-// Each wave takes task first + blockIdx.x without an atomic, then claims
-// further tasks from the queue counter.  Lane 0 stores the finished price and
-// claims the next task in one divergent region; every lane then reads the
-// claim back through a convergent broadcast, so all 64 lanes always leave the
-// loop together.
-template <class T, int KMAX, bool NARROW>
+// the waves work through tasks [first, last) of the task list.
+//   DYNAMIC (wide trees): each wave takes task first + blockIdx.x, then claims
+//     further tasks from the queue counter, so the long trees at the head of
+//     the list balance themselves.  Lane 0 stores the finished price and
+//     claims the next task in one divergent region; every lane then reads the
+//     claim back through a convergent broadcast, so all 64 lanes always leave
+//     the loop together.
+//   static (narrow trees): wave b takes tasks first + b, first + b + grid, ...
+//     Narrow tasks are many and short, and one counter bumped once per task
+//     by every wave serialises in L2; the list is in decreasing work order, so
+//     a fixed stride spreads the work evenly.
+template <class T, int KMAX, bool NARROW, bool DYNAMIC>
 __device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                             const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                             int queue, int first, int last, double *__restrict__ out,
@@ -309,12 +352,17 @@ __device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, co
         } else {
             p = price_option<T, KMAX>(c, eo, lane, (typename Lds<T>::type *)lds, (lds_f64 *)etab, fscratch);
         }
-        int claim = 0;
-        if (lane == 0) {
-            out[o] = p;
-            claim = atomicAdd(&tally[queue], 1);
+        if (DYNAMIC) {
+            int claim = 0;
+            if (lane == 0) {
+                out[o] = p;
+                claim = atomicAdd(&tally[queue], 1);
+            }
+            t = uni(__shfl(claim, 0, WAVE)) + first + stride;
+        } else {
+            if (lane == 0) out[o] = p;
+            t += stride;
         }
-        t = uni(__shfl(claim, 0, WAVE)) + first + stride;
     }
 }
 
@@ -324,7 +372,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
 {
     const int nlarge = uni(tally[T_NLARGE]);
-    price_tasks<T, MAXK, false>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
+    price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
 }
 
 template <class T>
@@ -333,7 +381,7 @@ __global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restric
                                                        double *__restrict__ out)
 {
     const int nlarge = uni(tally[T_NLARGE]), ntask = uni(tally[T_NTASK]);
-    price_tasks<T, KSMALL, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
+    price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
 }
 
 // Form 1: one lane per option over every scheduled task (tasks ordered by
@@ -493,10 +541,10 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
     k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
-        const int blocks = (n + 255) / 256;
-        k_setup<<<blocks, 256, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->edge, p->bucket, p->hist, p->tally, out);
+        const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
+        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->edge, p->bucket, p->hist, p->tally, out);
         k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
-        k_scatter<<<blocks, 256, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
+        k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
             const int lb = p->lane_block;

@@ -107,6 +107,9 @@ def _one(pkg, **kw):
     dict(a=0.15, term_steps=1, maturity=8.0, length=3.0, sigma=0.03),    # jmax 2
     dict(a=0.08, term_steps=1, maturity=9.0, length=4.0, sigma=0.03),    # jmax 3
     dict(length=0.04),                                                   # exercise at step 0
+    dict(length=0.25),                                                   # exercise at step 3 < jmax 23
+    dict(length=1.0, type=0),                                            # exercise at step 12 < jmax 23
+    dict(a=0.02, term_steps=36, maturity=5.0, length=2.0),               # e 72 < jmax 256: cone narrows
     dict(length=8.95),                                                   # exercise at the last step
     dict(length=9.0),                                                    # exercise == height: bond
     dict(length=12.0),                                                   # beyond the tree: bond
