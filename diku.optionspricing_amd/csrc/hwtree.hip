@@ -26,6 +26,7 @@
 #include "hwt_lane.h"
 #include "hwt_wave.h"
 #include "../../include/hwtree.h"
+#include "hwt_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -44,11 +45,12 @@ using namespace hwt;
 namespace {
 
 struct DevOpts {
-    int32_t n;
+    int32_t n;  // options, or the capacity bound when `count` is set
     const double *strike, *maturity, *length;
     const uint16_t *term_unit, *term_steps;
     const double *a, *sigma;
     const int8_t *type;
+    const int32_t *count;  // device-side option count (<= n), or null
 };
 
 struct DevYield {
@@ -111,7 +113,8 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     if (tid < 2) lbad[tid] = 0;
     __syncthreads();
     const int i = tid + blockIdx.x * SETUP_BLOCK;
-    if (i < o.n) {
+    const int nopt = o.count ? min(*o.count, o.n) : o.n;
+    if (i < nopt) {
         OptConst c;
         const int32_t st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i],
                                             o.term_steps[i], o.a[i], o.sigma[i], o.type[i], maxWidth, c);
@@ -196,7 +199,8 @@ __global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ o
 // Option indices into the task list, bucket by bucket.  Ranks inside a block
 // come from LDS atomics; one global atomic per (block, bucket) reserves the
 // block's slots.  Order inside a bucket is arbitrary (options are independent).
-__global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_t *__restrict__ bucket,
+__global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_t *__restrict__ count,
+                                                         const int32_t *__restrict__ bucket,
                                                          const int32_t *__restrict__ offs,
                                                          int32_t *__restrict__ cursor, int32_t *__restrict__ order)
 {
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lcount[k] = 0;
     __syncthreads();
     const int i = tid + blockIdx.x * SETUP_BLOCK;
+    if (count) n = min(*count, n);
     const int b = (i < n) ? bucket[i] : -1;
     const int r = (b >= 0) ? atomicAdd(&lcount[b], 1) : 0;
     __syncthreads();
@@ -492,7 +497,7 @@ bool valid_yield_ptrs(const hwt_yield_f64 *y) { return y && y->n > 0 && y->rate 
 
 DevOpts dev_opts(const hwt_options_f64 &o)
 {
-    return DevOpts{o.n, o.strike, o.maturity, o.length, o.term_unit, o.term_steps, o.a, o.sigma, o.type};
+    return DevOpts{o.n, o.strike, o.maturity, o.length, o.term_unit, o.term_steps, o.a, o.sigma, o.type, nullptr};
 }
 
 } // namespace
@@ -544,7 +549,7 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
         k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->edge, p->bucket, p->hist, p->tally, out);
         k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
-        k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, p->bucket, p->offs, p->cursor, p->order);
+        k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
             const int lb = p->lane_block;
@@ -772,6 +777,33 @@ int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_
         HWT_HIP(hipGraphLaunch(p->exec, s));
         p->timed_kernel = p->timed_split = false;
     }
+    HWT_HIP(hipEventRecord(p->ev1, s));
+    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    return HWT_OK;
+}
+
+int hwt_internal_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
+
+// In-stream variant for callers that only know the option count on the
+// device (the surrogate's fallback list): arrays of dopts->n (the bound)
+// options, of which the first *dev_count are priced.  Launched directly on
+// `stream`, never through the plan's graph.
+int hwt_plan_execute_counted(hwt_plan *p, const hwt_options_f64 *dopts, const int32_t *dev_count,
+                             const hwt_yield_f64 *dyield, double *dout, void *stream)
+{
+    g_last_error.clear();
+    if (!p || !dev_count || !valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
+        return fail(HWT_EINVAL, "hwt_plan_execute_counted: null or negative-size argument");
+    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "hwt_plan_execute_counted: bound above the plan capacity");
+    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "hwt_plan_execute_counted: yield curve longer than the plan capacity");
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    DevOpts o = dev_opts(*dopts);
+    o.count = dev_count;
+    const DevYield y{dyield->n, dyield->rate, dyield->day};
+    p->last_stream = s;
+    HWT_HIP(hipEventRecord(p->ev0, s));
+    const int rc = launch_pipeline(p, o, y, dout, s, true);
+    if (rc) return rc;
     HWT_HIP(hipEventRecord(p->ev1, s));
     HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     return HWT_OK;

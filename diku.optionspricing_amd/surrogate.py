@@ -1,0 +1,196 @@
+"""ctypes binding of the surrogate C ABI (include/hwtsurr.h), in the same
+library as the tree pricer.  No CPU fallback: every call goes to the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import OptionsC, YieldC, _check, lib
+
+NPARAMS = 199937
+HIDDEN = 256
+INPUTS = 8
+MAX_MEMBERS = 4
+RESIDUAL_SCALE = 0.01
+OFFSETS = {"W1": 0, "b1": 2048, "W2": 2304, "b2": 67840, "W3": 68096, "b3": 133632,
+           "W4": 133888, "b4": 199424, "W5": 199680, "b5": 199936}
+SHAPES = {"W1": (256, 8), "b1": (256,), "W2": (256, 256), "b2": (256,), "W3": (256, 256), "b3": (256,),
+          "W4": (256, 256), "b4": (256,), "W5": (1, 256), "b5": (1,)}
+
+EXPORTS = (
+    "hwt_surr_config_init", "hwt_surr_create", "hwt_surr_destroy", "hwt_surr_set_training_data",
+    "hwt_surr_train", "hwt_surr_predict", "hwt_surr_price", "hwt_surr_get_params",
+    "hwt_surr_set_params", "hwt_surr_eval_rows", "hwt_surr_get_inputs", "hwt_surr_adam_step",
+    "hwt_surr_save", "hwt_surr_load",
+)
+
+
+class SurrConfigC(C.Structure):
+    _fields_ = [("members", C.c_int32), ("batch", C.c_int32), ("lr_max", C.c_float), ("lr_min", C.c_float),
+                ("warmup_frac", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("spread_threshold", C.c_float), ("reserved0", C.c_int32), ("seed", C.c_uint64),
+                ("reserved", C.c_int32 * 6)]
+
+
+class SurrStatsC(C.Structure):
+    _fields_ = [("ms", C.c_double), ("loss", C.c_double), ("steps", C.c_int64), ("fallback", C.c_int32),
+                ("out_of_box", C.c_int32), ("surrogate_ms", C.c_double), ("tree_ms", C.c_double),
+                ("reserved", C.c_int32 * 4)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = lib()
+    if not _bound:
+        P = C.c_void_p
+        L.hwt_surr_config_init.argtypes = [C.POINTER(SurrConfigC)]
+        L.hwt_surr_config_init.restype = None
+        L.hwt_surr_create.argtypes = [C.POINTER(P), C.c_int32, C.c_int32, C.c_int32, C.POINTER(SurrConfigC)]
+        L.hwt_surr_destroy.argtypes = [P]
+        L.hwt_surr_destroy.restype = None
+        L.hwt_surr_set_training_data.argtypes = [P, C.POINTER(OptionsC), C.POINTER(YieldC), P]
+        L.hwt_surr_train.argtypes = [P, C.c_int32, C.c_int32, C.POINTER(SurrStatsC)]
+        L.hwt_surr_predict.argtypes = [P, C.POINTER(OptionsC), C.POINTER(YieldC), P, P, P]
+        L.hwt_surr_price.argtypes = [P, P, C.POINTER(OptionsC), C.POINTER(YieldC), P, C.POINTER(SurrStatsC)]
+        L.hwt_surr_get_params.argtypes = [P, P]
+        L.hwt_surr_set_params.argtypes = [P, P]
+        L.hwt_surr_eval_rows.argtypes = [P, C.c_int32, P, C.c_int32, P, P, P]
+        L.hwt_surr_get_inputs.argtypes = [P, P, C.c_int32, P, P]
+        L.hwt_surr_adam_step.argtypes = [P, P, C.c_float]
+        L.hwt_surr_save.argtypes = [P, C.c_char_p]
+        L.hwt_surr_load.argtypes = [P, C.c_char_p]
+        _bound = True
+    return L
+
+
+def config(**kw) -> SurrConfigC:
+    c = SurrConfigC()
+    _lib().hwt_surr_config_init(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _dev_opts(dev: dict, n: int | None = None) -> OptionsC:
+    n = int(dev["n"] if n is None else n)
+    return OptionsC(n, *[dev[f].value for f in capi.Plan.FIELDS])
+
+
+def _dev_yield(dev: dict) -> YieldC:
+    return YieldC(int(dev["ny"]), dev["rates"].value, dev["days"].value)
+
+
+def split_params(flat: np.ndarray) -> dict:
+    """One member's flat parameter vector -> named tensors (views)."""
+    return {k: flat[o:o + int(np.prod(SHAPES[k]))].reshape(SHAPES[k]) for k, o in OFFSETS.items()}
+
+
+class Surrogate:
+    """An ensemble surrogate on the GPU (hwt_surrogate)."""
+
+    def __init__(self, train_capacity: int, infer_capacity: int, yield_capacity: int,
+                 cfg: SurrConfigC | None = None):
+        self.cfg = cfg if cfg is not None else config()
+        self._s = C.c_void_p()
+        _check(_lib().hwt_surr_create(C.byref(self._s), int(train_capacity), int(infer_capacity),
+                                      int(yield_capacity), C.byref(self.cfg)))
+
+    @property
+    def members(self) -> int:
+        return int(self.cfg.members)
+
+    def set_training_data(self, dev: dict, prices):
+        """dev: device options/yield (capi.to_device); prices: device FP64 tree prices."""
+        _check(_lib().hwt_surr_set_training_data(self._s, C.byref(_dev_opts(dev)), C.byref(_dev_yield(dev)),
+                                                 prices.value))
+
+    def train(self, epochs: int, total_epochs: int) -> dict:
+        st = SurrStatsC()
+        _check(_lib().hwt_surr_train(self._s, int(epochs), int(total_epochs), C.byref(st)))
+        return st.as_dict()
+
+    def predict(self, dev: dict):
+        """Ensemble-mean price, spread and in-box flag per option (host arrays)."""
+        from .devmem import DeviceArray
+        n = int(dev["n"])
+        mean, spread = DeviceArray.empty((n,), np.float64), DeviceArray.empty((n,), np.float64)
+        inbox = DeviceArray.empty((n,), np.int8)
+        _check(_lib().hwt_surr_predict(self._s, C.byref(_dev_opts(dev)), C.byref(_dev_yield(dev)),
+                                       mean.value, spread.value, inbox.value))
+        return mean.to_host(), spread.to_host(), inbox.to_host().astype(bool)
+
+    def price(self, tree: capi.Plan, dev: dict, out=None) -> dict:
+        """Surrogate + tree fallback into out (default dev['out'])."""
+        st = SurrStatsC()
+        out = dev["out"] if out is None else out
+        _check(_lib().hwt_surr_price(self._s, tree._p, C.byref(_dev_opts(dev)), C.byref(_dev_yield(dev)),
+                                     out.value, C.byref(st)))
+        return st.as_dict()
+
+    def get_params(self) -> np.ndarray:
+        P = np.empty((self.members, NPARAMS), np.float32)
+        _check(_lib().hwt_surr_get_params(self._s, P.ctypes.data))
+        return P
+
+    def set_params(self, P: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.float32).reshape(self.members, NPARAMS)
+        _check(_lib().hwt_surr_set_params(self._s, P.ctypes.data))
+
+    def eval_rows(self, member: int, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        out = np.empty(rows.size, np.float32)
+        grad = np.empty(NPARAMS, np.float32)
+        loss = C.c_float()
+        _check(_lib().hwt_surr_eval_rows(self._s, int(member), rows.ctypes.data, rows.size, out.ctypes.data,
+                                         C.byref(loss), grad.ctypes.data))
+        return out, float(loss.value), grad
+
+    def get_inputs(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        x = np.empty((rows.size, INPUTS), np.float32)
+        t = np.empty(rows.size, np.float32)
+        _check(_lib().hwt_surr_get_inputs(self._s, rows.ctypes.data, rows.size, x.ctypes.data, t.ctypes.data))
+        return x, t
+
+    def adam_step(self, grad: np.ndarray, lr: float):
+        g = np.ascontiguousarray(grad, dtype=np.float32).reshape(self.members, NPARAMS)
+        _check(_lib().hwt_surr_adam_step(self._s, g.ctypes.data, float(lr)))
+
+    def save(self, path):
+        _check(_lib().hwt_surr_save(self._s, str(path).encode()))
+
+    def load(self, path):
+        _check(_lib().hwt_surr_load(self._s, str(path).encode()))
+
+    def close(self):
+        if self._s:
+            _lib().hwt_surr_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def training_set(seed: int, n: int, y, tree_plan_capacity: int | None = None):
+    """Box samples priced by the FP64 tree on the GPU: (options, device
+    buffers, device prices).  The surrogate's training data generator."""
+    from . import datasets
+    opts = datasets.box(seed, n)
+    dev = capi.to_device(opts, y)
+    plan = capi.Plan(tree_plan_capacity or n, y.N)
+    plan.execute(dev)
+    plan.sync()
+    plan.close()
+    return opts, dev, dev["out"]

@@ -39,7 +39,7 @@ extern "C" {
 #define HWT_EHIP         -4  /* HIP runtime error */
 
 /* Kernel forms (the reference's two CUDA runners plus an inspector). */
-#define HWT_FORM_AUTO 0  /* per option: lane form for tiny trees, wave form otherwise */
+#define HWT_FORM_AUTO 0  /* the fastest form for the batch: currently the wave form */
 #define HWT_FORM_LANE 1  /* form 1: one lane per option, columns interleaved in HBM
                             (replaces cuda-option, reference cuda-option/Kernel.cuh:65-216) */
 #define HWT_FORM_WAVE 2  /* form 2: columns on chip, many options per workgroup

@@ -4,13 +4,14 @@ the host-only entry points behave."""
 import ctypes as C
 import re
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
 
 
-def _declared_functions():
-    text = (ROOT / "include" / "hwtree.h").read_text()
+def _declared_functions(header="hwtree.h"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(hwt_\w+)\s*\(", text)))
 
@@ -27,6 +28,10 @@ def test_library_exports_every_declared_symbol(pkg):
     missing = [n for n in _declared_functions() if not hasattr(lib, n)]
     assert not missing, f"not exported: {missing}"
     assert set(pkg.capi.EXPORTS) == set(_declared_functions())
+    surr = _declared_functions("hwtsurr.h")
+    missing = [n for n in surr if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    assert set(pkg.surrogate.EXPORTS) == set(surr)
 
 
 def test_host_only_entry_points(pkg):
@@ -46,6 +51,24 @@ def test_struct_layouts_match_header(pkg):
     assert C.sizeof(pkg.capi.YieldC) == 24
     assert C.sizeof(pkg.capi.ConfigC) == 8 * 4
     assert C.sizeof(pkg.capi.StatsC) == 8 * 8 + 4 * 4
+    assert C.sizeof(pkg.surrogate.SurrConfigC) == 10 * 4 + 8 + 6 * 4
+    assert C.sizeof(pkg.surrogate.SurrStatsC) == 3 * 8 + 2 * 4 + 2 * 8 + 4 * 4
+
+
+def test_surrogate_host_only_entry_points(pkg):
+    lib = pkg.surrogate._lib()
+    cfg = pkg.surrogate.config()
+    assert cfg.members == 4 and cfg.batch == 8192 and abs(cfg.spread_threshold - 1e-3) < 1e-9
+    assert lib.hwt_surr_create(None, 10, 10, 10, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_surr_train(None, 1, 1, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_surr_price(None, None, None, None, None, None) == pkg.capi.HWT_EINVAL
+    lib.hwt_surr_destroy(None)  # no-op
+    # parameter offsets of the header
+    text = (ROOT / "include" / "hwtsurr.h").read_text()
+    for name, off in pkg.surrogate.OFFSETS.items():
+        assert re.search(rf"#define HWT_SURR_{name.upper()}\s+{off}\b", text), name
+    sizes = {k: int(np.prod(v)) for k, v in pkg.surrogate.SHAPES.items()}
+    assert sum(sizes.values()) == pkg.surrogate.NPARAMS == 199937
 
 
 def test_null_arguments_rejected_without_gpu(pkg):
