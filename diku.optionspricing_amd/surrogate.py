@@ -4,6 +4,7 @@ library as the tree pricer.  No CPU fallback: every call goes to the GPU.
 from __future__ import annotations
 
 import ctypes as C
+import json
 
 import numpy as np
 
@@ -194,3 +195,105 @@ def training_set(seed: int, n: int, y, tree_plan_capacity: int | None = None):
     plan.sync()
     plan.close()
     return opts, dev, dev["out"]
+
+
+def fit(y, n_train: int = 2_000_000, epochs: int = 40, seed: int = 11, infer_capacity: int = 1 << 20,
+        cfg: SurrConfigC | None = None, log=None):
+    """Trains an ensemble on `n_train` box samples priced by the FP64 tree:
+    (surrogate, record).  The record holds the data-generation time, the
+    per-epoch losses and the total training time (GPU, HIP events)."""
+    import time
+    from . import devmem
+    t0 = time.perf_counter()
+    opts, dev, prices = training_set(seed, n_train, y)
+    devmem.synchronize()
+    t_data = time.perf_counter() - t0
+    s = Surrogate(n_train, infer_capacity, y.N, cfg)
+    s.set_training_data(dev, prices)
+    losses, train_ms = [], 0.0
+    for ep in range(epochs):
+        st = s.train(1, epochs)
+        losses.append(st["loss"])
+        train_ms += st["ms"]
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {st['loss']:.4e}, {st['ms']:.0f} ms")
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()
+    rec = {"samples": n_train, "epochs": epochs, "members": s.members, "batch": int(s.cfg.batch),
+           "data_s": t_data, "train_s": train_ms * 1e-3, "steps": epochs * (n_train // int(s.cfg.batch)),
+           "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}
+    return s, rec
+
+
+def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, log=None) -> dict:
+    """Surrogate + tree fallback against the plain FP64 tree on the mixed
+    batch (95% inside the box, 5% outside): options/s of each (HIP events),
+    the fallback fraction, and the error of every price against the tree's."""
+    from . import datasets, devmem
+    opts = datasets.box_mix(seed, n)
+    dev = capi.to_device(opts, y)
+
+    # plain tree: the plan's HIP graph, timed with events on a caller stream
+    stream = C.c_void_p()
+    devmem._ck(devmem.hip().hipStreamCreate(C.byref(stream)))
+    plan = capi.Plan(n, y.N)
+    plan.execute(dev, stream=stream.value)
+    plan.sync()
+    tree = dev["out"].to_host()
+    e0, e1 = devmem.Event(), devmem.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.execute(dev, stream=stream.value)
+    e1.record(stream)
+    e1.synchronize()
+    plan.sync()
+    tree_ms = e0.elapsed_ms(e1) / reps
+    plan.close()
+
+    fb_plan = capi.Plan(n, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+    out = devmem.DeviceArray.empty((n,), np.float64)
+    s.price(fb_plan, dev, out)  # warm-up
+    runs = [s.price(fb_plan, dev, out) for _ in range(reps)]
+    got = out.to_host()
+    fb_plan.close()
+    mean, spread, inbox = s.predict(dev)
+    # the method without the network: the closed-form baseline alone (an
+    # ensemble whose output layer is zero predicts exactly the baseline)
+    P0 = s.get_params()
+    P0[:, OFFSETS["W5"]:] = 0.0
+    s0 = Surrogate(0, n, y.N, config(members=s.members))
+    s0.set_params(P0)
+    base, _, _ = s0.predict(dev)
+    s0.close()
+    devmem._ck(devmem.hip().hipStreamDestroy(stream))
+
+    ms = float(np.mean([r["ms"] for r in runs]))
+    err = np.abs(got - tree)
+    surr_rows = inbox & (spread <= s.cfg.spread_threshold)
+    err_net = np.abs(mean[inbox] - tree[inbox])
+    rec = {
+        "options": n,
+        "tree_ms": tree_ms, "tree_options_per_s": n / (tree_ms * 1e-3),
+        "surrogate_ms": ms, "surrogate_options_per_s": n / (ms * 1e-3),
+        "speedup": tree_ms / ms,
+        "inference_ms": float(np.mean([r["surrogate_ms"] for r in runs])),
+        "fallback_tree_ms": float(np.mean([r["tree_ms"] for r in runs])),
+        "fallback": int(runs[-1]["fallback"]), "out_of_box": int(runs[-1]["out_of_box"]),
+        "fallback_frac": runs[-1]["fallback"] / n,
+        "max_abs_err": float(err.max()), "mean_abs_err": float(err.mean()),
+        "max_abs_err_surrogate_priced": float(err[surr_rows].max()) if surr_rows.any() else 0.0,
+        "in_box_network_max_abs_err": float(err_net.max()) if err_net.size else 0.0,
+        "in_box_network_mean_abs_err": float(err_net.mean()) if err_net.size else 0.0,
+        "spread_threshold": float(s.cfg.spread_threshold),
+        "options_above_1e-3": int((err > 1e-3).sum()),
+        "in_box_closed_form_max_abs_err": float(np.abs(base[inbox] - tree[inbox]).max()) if inbox.any() else 0.0,
+        "in_box_closed_form_mean_abs_err": float(np.abs(base[inbox] - tree[inbox]).mean()) if inbox.any() else 0.0,
+    }
+    if log:
+        log(json.dumps(rec))
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()
+    out.free()
+    return rec
