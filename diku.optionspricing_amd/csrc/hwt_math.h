@@ -25,7 +25,8 @@ struct OptConst {
     double Pe, Pn;         // discount bonds P(e dt), P(n dt) from the yield curve
     double rho, rhop;      // edge-elimination factors (bottom, top)
     int32_t jmax, n, e, w; // half width, height, exercise step (-1: none), width
-    int32_t type, status;  // CALL 0 / PUT 1, ST_*
+    int16_t type, unit;    // CALL 0 / PUT 1; 1 if the unit-diagonal stencil form is safe
+    int32_t status;        // ST_*
     int64_t work;          // stencil node updates this option needs
 };
 static_assert(sizeof(OptConst) == 96, "OptConst layout");
@@ -47,6 +48,27 @@ HWT_HD double yield_at(double t, int termUnit, const double *rates, const int32_
     if (first == size - 1) return rates[size - 1];
     const int t1 = days[first], t2 = days[second];
     const double p1 = rates[first], p2 = rates[second];
+    const double coef = (tDays - t1) / (double)(t2 - t1);
+    return p1 + coef * (p2 - p1);
+}
+
+// The same value by binary search, for curves whose days never decrease: the
+// linear scan above stops at the first knot with day >= tDays, which is the
+// lower bound; with no such knot it returns the last rate, and with the first
+// knot already past tDays the first rate (so one knot gives rates[0] either way).
+HWT_HD double yield_at_sorted(double t, int termUnit, const double *rates, const int32_t *days, int size)
+{
+    const int tDays = (int)round(t * termUnit);
+    int lo = 0, hi = size;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (days[mid] >= tDays) hi = mid;
+        else lo = mid + 1;
+    }
+    if (lo == 0) return rates[0];
+    if (lo == size) return rates[size - 1];
+    const int t1 = days[lo - 1], t2 = days[lo];
+    const double p1 = rates[lo - 1], p2 = rates[lo];
     const double coef = (tDays - t1) / (double)(t2 - t1);
     return p1 + coef * (p2 - p1);
 }
@@ -188,6 +210,7 @@ HWT_HD int32_t option_constants(double strike, double T, double len, uint16_t te
 {
     c.X = strike;
     c.type = type;
+    c.unit = 0;
     c.Pe = c.Pn = c.rho = c.rhop = 0.0;
     c.work = 0;
     c.n = c.jmax = c.w = 0;

@@ -62,6 +62,17 @@ __device__ __forceinline__ double wave_sum(double v)
     return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
+// Sum over each 16-lane row, the same in every lane of the row: the first
+// four butterflies of wave_sum.
+__device__ __forceinline__ double row_sum(double v)
+{
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+
 // Scalar (wave-uniform) copies of values the caller guarantees are uniform.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ double uni(double v)
@@ -308,12 +319,174 @@ __device__ __forceinline__ void init_window(T (&z)[K], const TreeShape &t, int b
     }
 }
 
+// ------------------------------------------------- unit-diagonal stencil
+// The diagonal similarity u = D^{-1} z, D = diag(d), with d = 1 on each lane's
+// first node and d_k = G[k][k-1] d_{k-1} along the lane, makes every in-lane
+// sub-diagonal entry 1, so a node update costs two FMAs instead of three ops:
+//     u'_k = u_{k-1} + cm_k u_k + cr'_k u_{k+1}           (s >= 1)
+//     u'_k = cl'_k u_{k-1} + cm_k u_k + cr'_k u_{k+1}     (s = 0)
+// with cr'_k = cr_k d_{k+1} / d_k and cl'_k = cl_k d_{k-1} (d_{k-1} is the
+// lower lane's last scale).  Padding nodes (k >= w) take d_k = d_{k-1}: their
+// values become copies of real values, which real nodes read only through
+// zero coefficients.  The setup kernel checks per option that every scale
+// stays in range (unit_form_ok); the plain stencil prices the rest.
+//
+// The plain coefficients come from window_coefs (edge nodes patched by a
+// wave-uniform switch, so register indices stay static) and are transformed
+// in place; z is scaled with one division per lane.  Undoing D after the
+// steps needs the scales again: in FP64 each lane parks them in its own node
+// slots of the LDS state buffer, which the steps leave untouched; in mixed
+// mode (an FP32 buffer) they are recomputed.
+
+// Transformed coefficients of this lane's nodes, and z -> D^{-1} z: the chain
+// d_s = c_s d_{s-1} is built going up, then walked down with
+// 1/d_{s-1} = c_s / d_s.
+template <class T, int K, bool FWD>
+__device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, const lds_f64 *etab, T (&z)[K],
+                                        T &cl0, T (&um)[K], T (&ur)[K], typename Lds<T>::type *lds)
+{
+    double cl[K], cm[K], cr[K];
+    window_coefs<double, K, FWD>(t, base, lane, etab, cl, cm, cr);
+    const int k0 = base + lane * K;
+    double d = 1.0;
+#pragma unroll
+    for (int s = 1; s < K; ++s) {
+        cl[s] = (k0 + s < t.w) ? cl[s] : 1.0;
+        d *= cl[s];
+        cr[s - 1] *= cl[s];
+        if constexpr (sizeof(T) == sizeof(double)) lds[k0 + s] = d;
+    }
+    double inv = 1.0 / d;
+    cr[K - 1] *= inv;
+    cl0 = (T)(cl[0] * dpp_from_lower_lane(d));
+#pragma unroll
+    for (int s = K - 1; s > 0; --s) {
+        z[s] = (T)((double)z[s] * inv);
+        inv *= cl[s];
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        um[s] = (T)cm[s];
+        ur[s] = (T)cr[s];
+    }
+}
+
+// z -> D z with the scales unit_in used.  When they are recomputed, the
+// window base passes through an empty asm so the compiler cannot merge this
+// with unit_in's computation and keep that alive across the steps.
+template <class T, int K, bool FWD>
+__device__ __forceinline__ void unit_out(const TreeShape &t, int base, int lane, const lds_f64 *etab, T (&z)[K],
+                                         typename Lds<T>::type *lds)
+{
+    const int k0 = base + lane * K;
+    if constexpr (sizeof(T) == sizeof(double)) {
+#pragma unroll
+        for (int s = 1; s < K; ++s) z[s] = (T)((double)z[s] * lds[k0 + s]);
+    } else {
+        asm volatile("" : "+s"(base));
+        double cl[K], cm[K], cr[K];
+        window_coefs<double, K, FWD>(t, base, lane, etab, cl, cm, cr);
+        double d = 1.0;
+#pragma unroll
+        for (int s = 1; s < K; ++s) {
+            d *= (base + lane * K + s < t.w) ? cl[s] : 1.0;
+            z[s] = (T)((double)z[s] * d);
+        }
+    }
+}
+
+// Largest window the unit form is used for.  FP64: beyond 12 nodes per lane
+// the transform's coefficient setup no longer fits the register budget of two
+// waves per SIMD; windows of K <= 12 carry about 80% of the stencil work of a
+// typical wide batch.  FP32 (mixed mode): not used -- the undiscounted
+// backward state of a long, weakly mean-reverting bond already comes within a
+// few powers of ten of the FP32 limit (about 1e34 for a 100-year bond with
+// a = 0.009), and D^{-1} would scale it past.
+template <class T> struct UnitMaxK;
+template <> struct UnitMaxK<double> { static constexpr int value = 12; };
+template <> struct UnitMaxK<float> { static constexpr int value = 1; };
+
+// a*b + c into a fresh register.  Written as the three-address v_fma_f64:
+// left to itself the compiler accumulates into c's register (v_fmac) and then
+// spends a 64-bit move per node restoring the loop-carried registers.  The
+// result only feeds an ordinary FMA, which is what the DPP shifts read.
+__device__ __forceinline__ double fma3(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+__device__ __forceinline__ float fma3(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <class T, int K>
+__device__ __forceinline__ void unit_step(const T (&src)[K], T (&dst)[K], T cl0, const T (&cm)[K], const T (&cr)[K])
+{
+    const T left = dpp_from_lower_lane(src[K - 1]);
+    const T right = dpp_from_upper_lane(src[0]);
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const T nx = (s + 1 < K) ? src[s + 1] : right;
+        T v;
+        if (s == 0) {
+            v = cm[0] * src[0];
+            v = fma(cl0, left, v);
+        } else {
+            v = fma3(cm[s], src[s], src[s - 1]);
+        }
+        dst[s] = fma(cr[s], nx, v);
+    }
+}
+
+template <class T, int K>
+__device__ __forceinline__ void unit_steps(T (&z)[K], T cl0, const T (&cm)[K], const T (&cr)[K], int nsteps)
+{
+    T t[K];
+    for (int it = 0; it + 1 < nsteps; it += 2) {
+        unit_step<T, K>(z, t, cl0, cm, cr);
+        unit_step<T, K>(t, z, cl0, cm, cr);
+    }
+    if (nsteps & 1) {
+        unit_step<T, K>(z, t, cl0, cm, cr);
+#pragma unroll
+        for (int s = 0; s < K; ++s) z[s] = t[s];
+    }
+}
+
+// Range the unit form's scales must stay in: D^{-1} z and the transformed
+// coefficients then stay finite for any state below about 1e208.
+template <class T> struct UnitRange;
+template <> struct UnitRange<double> { static constexpr double dmin = 1e-100; };
+template <> struct UnitRange<float> { static constexpr double dmin = 1e-30; };
+
+// Whether every scale d of every window of this option lies in
+// [dmin, 1/dmin]: a scale is a product of at most K-1 sub-diagonal entries
+// (K = ceil(w/64)), each interior entry is a standard probability
+// (between 1/6 - jmax|M|/2 and 1/6 + (jmax^2 M^2 + jmax|M|)/2) times a node
+// discount in [e^{-X}, e^{X}], X = jmax dr dt, and the edge entries are the
+// table's sub-diagonals of nodes 1, 2, w-3, w-2, w-1 (node 0 always starts a
+// lane).
+HWT_HD bool unit_form_ok(const TreeShape &t, const double *tab, double dmin)
+{
+    const double jm = t.jmax * fabs(t.M), X = t.jmax * t.dr * t.dt;
+    double lo = (1.0 / 6.0 - 0.5 * jm) * exp(-X), hi = (1.0 / 6.0 + 0.5 * (jm * jm + jm)) * exp(X);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int slot = 1; slot < 6; ++slot) {
+            const double a = fabs(tab[18 * pass + 3 * slot]);
+            lo = a < lo ? a : lo;
+            hi = a > hi ? a : hi;
+        }
+    if (!(lo > 0.0) || !(hi < 1e300)) return false;
+    const int f = (t.w + WAVE - 1) / WAVE - 1;  // most factors in one lane
+    const double L = f * log(lo < 1.0 ? lo : 1.0), U = f * log(hi > 1.0 ? hi : 1.0);
+    return L >= log(dmin) && U <= -log(dmin);
+}
+
 // One phase: bring the window in (initial values, or the previous window's
 // nodes [in_lo, in_hi) from LDS, zeros elsewhere), run nsteps stencil steps,
 // and leave the window's tree nodes in LDS, or in `out` when given.
 template <class T, int K, bool FWD>
 __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nsteps, int in_lo, int in_hi, int lane,
-                                         typename Lds<T>::type *lds, const lds_f64 *etab, T *out)
+                                         typename Lds<T>::type *lds, const lds_f64 *etab, T *out, bool unit)
 {
     T z[K];
     if (in_lo >= in_hi) {
@@ -327,9 +500,16 @@ __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nstep
     }
     __syncthreads();
     if (nsteps > 0) {
-        T cl[K], cm[K], cr[K];
-        window_coefs<T, K, FWD>(t, base, lane, etab, cl, cm, cr);
-        stencil_steps<T, K>(z, cl, cm, cr, nsteps);
+        if (K > 1 && K <= UnitMaxK<T>::value && unit) {
+            T cl0, um[K], ur[K];
+            unit_in<T, K, FWD>(t, base, lane, etab, z, cl0, um, ur, lds);
+            unit_steps<T, K>(z, cl0, um, ur, nsteps);
+            unit_out<T, K, FWD>(t, base, lane, etab, z, lds);
+        } else {
+            T cl[K], cm[K], cr[K];
+            window_coefs<T, K, FWD>(t, base, lane, etab, cl, cm, cr);
+            stencil_steps<T, K>(z, cl, cm, cr, nsteps);
+        }
     }
     if (out) {
 #pragma unroll
@@ -352,13 +532,14 @@ __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nstep
 
 template <class T, int KMAX, bool FWD>
 __device__ __forceinline__ void dispatch_phase(int K, const TreeShape &t, int base, int nsteps, int in_lo, int in_hi,
-                                               int lane, typename Lds<T>::type *lds, const lds_f64 *etab, T *out)
+                                               int lane, typename Lds<T>::type *lds, const lds_f64 *etab, T *out,
+                                               bool unit)
 {
     switch (K) {
 #define HWT_C(KK)                                                                                        \
     case KK:                                                                                             \
         if (KK <= KMAX)                                                                                  \
-            phase_fn<T, (KK <= KMAX ? KK : 1), FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab, out); \
+            phase_fn<T, (KK <= KMAX ? KK : 1), FWD>(t, base, nsteps, in_lo, in_hi, lane, lds, etab, out, unit); \
         break;
         HWT_KCASES(HWT_C)
 #undef HWT_C
@@ -383,7 +564,7 @@ __device__ __forceinline__ Window window_for(int D, const TreeShape &t)
 // out[lo, hi) (global scratch); tree nodes outside that range are zero.
 template <class T, int KMAX>
 __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, typename Lds<T>::type *lds,
-                                             const lds_f64 *etab, T *out, int &lo, int &hi)
+                                             const lds_f64 *etab, T *out, int &lo, int &hi, bool unit)
 {
     Window win = window_for(0, t);
     int in_lo = 0, in_hi = 0;  // empty: initial values
@@ -393,7 +574,7 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
         const int iend = full ? e : min(e, 32 * win.K - 1);
         const bool last = iend >= e;
         dispatch_phase<T, KMAX, true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab,
-                                      last ? out : nullptr);
+                                      last ? out : nullptr, unit);
         i = iend;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -409,7 +590,7 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
 // state ends up in lds[lo, hi).
 template <class T, int KMAX>
 __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, typename Lds<T>::type *lds,
-                                              const lds_f64 *etab, int &lo, int &hi)
+                                              const lds_f64 *etab, int &lo, int &hi, bool unit)
 {
     const int ep = min(e, t.jmax);
     const int Kgmax = t.jmax / 32;
@@ -420,7 +601,8 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
         const bool full = 32 * win.K > t.jmax;
         const int klo = full ? Kgmax : win.K - 1;
         const int send = max(e, e + 32 * klo - 1 - ep);
-        dispatch_phase<T, KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, (T *)nullptr);
+        dispatch_phase<T, KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, (T *)nullptr,
+                                       unit);
         s = send;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -436,6 +618,7 @@ struct PlainSum {
     double s = 0.0;
     __device__ __forceinline__ void add(double x) { s += x; }
     __device__ __forceinline__ double wave_total() const { return wave_sum(s); }
+    __device__ __forceinline__ double row_total() const { return row_sum(s); }
 };
 
 struct KahanSum {
@@ -447,6 +630,7 @@ struct KahanSum {
         s = t;
     }
     __device__ __forceinline__ double wave_total() const { return wave_sum(s) + wave_sum(c); }
+    __device__ __forceinline__ double row_total() const { return row_sum(s) + row_sum(c); }
 };
 
 template <class T> struct SumOf { typedef PlainSum type; };

@@ -24,6 +24,7 @@
 // So each option costs n stencil steps (e forward + n-e backward) with no
 // per-step reductions, against 2n steps plus n reductions in seq/Seq.hpp.
 #include "hwt_lane.h"
+#include "hwt_rows.h"
 #include "hwt_wave.h"
 #include "../../include/hwtree.h"
 #include "hwt_internal.h"
@@ -59,11 +60,20 @@ struct DevYield {
     const int32_t *day;
 };
 
-constexpr int NBUCKET = 256;
+// Task classes, in task-list order:
+//   wide  (w > 64*KSMALL)           k_price_wide, one option per wave, LPT queue
+//   mid   (ROW_MAX_WIDTH < w <= 64*KSMALL)  k_price_narrow, one option per wave
+//   row   (w <= ROW_MAX_WIDTH)      k_price_narrow, four options per wave
+// Bucket ranges: wide [B_WIDE, 1024) and mid [B_MID, B_WIDE), log-spaced by
+// the sort key (work unless a width/height order is asked for); row
+// [0, B_MID), by (nodes per lane, height) so the four rows of a wave share
+// their step count.  The scan walks each class from its largest key down.
+constexpr int NBUCKET = 1024;
+constexpr int B_MID = 640, B_WIDE = 768;
 constexpr int KSMALL = 2;  // trees up to 64*KSMALL nodes wide go to the narrow-tree kernel
 enum Tally {
     T_QUEUE = 0,    // next task of the wide-tree kernel (tasks [0, T_NLARGE))
-    T_QUEUE_S = 1,  // next task of the narrow-tree kernel (tasks [T_NLARGE, T_NTASK))
+    T_QUEUE_S = 1,  // (unused: the narrow-tree kernel strides statically)
     T_NTASK = 2,
     T_NLARGE = 3,
     T_INVALID = 4,
@@ -71,16 +81,24 @@ enum Tally {
     T_WORK = 6,     // int64
     T_SEQWORK = 8,  // int64
     T_WORKW = 10,   // int64, wide trees only
-    T_NCOUNT = 12
+    T_NMID = 12,    // end of the mid-class tasks
+    T_NPAD = 13,    // idle slots padding the row class
+    T_NCOUNT = 14
 };
 
-// Buckets [128, 256) hold wide trees, [0, 128) narrow ones; the descending
-// scan puts every wide tree ahead of every narrow one in the task list.
-__device__ __forceinline__ int bucket_of(int64_t key, bool wide)
+__device__ __forceinline__ int log_bucket(int64_t key, int per_octave, int nb)
 {
-    int b = (int)(4.0 * log2((double)key + 1.0));
-    b = b < 0 ? 0 : (b > 127 ? 127 : b);
-    return wide ? 128 + b : b;
+    int b = (int)(per_octave * log2((double)key + 1.0));
+    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+// Row class: 160 height buckets per nodes-per-lane count K (1..4): exact
+// heights below 96, then 16 buckets per octave.
+__device__ __forceinline__ int row_bucket(int w, int n)
+{
+    const int K = (w + ROW - 1) / ROW;
+    const int h = n < 96 ? n : min(159, 96 + (int)(16.0 * log2(n / 96.0)));
+    return (K - 1) * 160 + h;
 }
 
 __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
@@ -98,8 +116,10 @@ __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 // global atomics number a few per block instead of one per option (single
 // counters hit by every option serialise in L2).
 constexpr int SETUP_BLOCK = 256;
+constexpr int SETUP_YMAX = 2048;  // knots staged in LDS by k_setup (24 KB)
 
 __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
+                                                       double unit_dmin,
                                                        OptConst *__restrict__ cst, double *__restrict__ edge,
                                                        int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
                                                        int32_t *__restrict__ tally, double *__restrict__ out)
@@ -107,11 +127,28 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     __shared__ int32_t lh[NBUCKET];
     __shared__ unsigned long long lsum[3];  // work, seq work, wide work
     __shared__ int32_t lbad[2];             // invalid, unsupported
+    __shared__ double yr[SETUP_YMAX];       // the yield curve, staged once per block
+    __shared__ int32_t yd[SETUP_YMAX];
     const int tid = threadIdx.x;
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lh[k] = 0;
     if (tid < 3) lsum[tid] = 0ull;
     if (tid < 2) lbad[tid] = 0;
-    __syncthreads();
+    // Curves that fit are read from LDS, by binary search when their days
+    // never decrease (every real curve); anything else keeps the reference's
+    // linear scan over global memory.  Both give the same value.
+    const bool staged = y.n <= SETUP_YMAX;
+    int unsorted = 0;
+    if (staged) {
+        for (int k = tid; k < y.n; k += SETUP_BLOCK) {
+            yr[k] = y.rate[k];
+            yd[k] = y.day[k];
+            if (k + 1 < y.n && y.day[k] > y.day[k + 1]) unsorted = 1;
+        }
+    }
+    const bool sorted = staged && !__syncthreads_or(unsorted);
+    auto yield = [&](double t, int tu) {
+        return sorted ? yield_at_sorted(t, tu, yr, yd, y.n) : yield_at(t, tu, y.rate, y.day, y.n);
+    };
     const int i = tid + blockIdx.x * SETUP_BLOCK;
     const int nopt = o.count ? min(*o.count, o.n) : o.n;
     if (i < nopt) {
@@ -131,17 +168,18 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
         } else {
             const int tu = o.term_unit[i];
             const double tn = c.n * c.dt;
-            c.Pn = exp(-yield_at(tn, tu, y.rate, y.day, y.n) * tn);
+            c.Pn = exp(-yield(tn, tu) * tn);
             if (c.e < 0) {
                 out[i] = 100.0 * c.Pn;  // exercise outside the tree: the zero-coupon bond
             } else {
                 const double te = c.e * c.dt;
-                c.Pe = exp(-yield_at(te, tu, y.rate, y.day, y.n) * te);
+                c.Pe = exp(-yield(te, tu) * te);
                 TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
                 edge_factors(t);
                 c.rho = t.rho;
                 c.rhop = t.rhop;
                 edge_table(t, edge + (size_t)i * EDGE_TAB);
+                c.unit = unit_form_ok(t, edge + (size_t)i * EDGE_TAB, unit_dmin) ? 1 : 0;
                 c.work = dual_node_updates(c.n, c.e, c.jmax);
                 const int64_t seqw = seq_node_updates(c.n, c.jmax);
                 atomicAdd(&lsum[0], (unsigned long long)c.work);
@@ -154,7 +192,9 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 default: key = c.work; break;
                 }
                 const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
-                b = bucket_of(key, wide);
+                if (wide) b = B_WIDE + log_bucket(key, 8, NBUCKET - B_WIDE);
+                else if (c.w > ROW_MAX_WIDTH) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
+                else b = row_bucket(c.w, c.n);
                 atomicAdd(&lh[b], 1);
                 if (wide) atomicAdd(&lsum[2], (unsigned long long)c.work);
             }
@@ -174,15 +214,33 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     }
 }
 
-// Exclusive offsets of the buckets: every wide-tree bucket (128..255) before
-// every narrow-tree bucket (0..127), each class largest key first unless an
-// ascending order was asked for.
-__global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ offs, int32_t *__restrict__ tally, int32_t asc)
+// Exclusive offsets of the buckets (one thread per bucket): the wide class,
+// then the mid class, each largest key first unless an ascending order was
+// asked for, then the row class by decreasing (K, height).  Each K part of
+// the row class is padded to a multiple of four tasks with idle slots
+// (order -1), so every group of four row tasks shares one K and an option's
+// arithmetic never depends on which options share its wave (not for the
+// lane form, which has no row groups).
+__global__ __launch_bounds__(NBUCKET) void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ offs,
+                                                  int32_t *__restrict__ tally, int32_t *__restrict__ order,
+                                                  int32_t asc, int32_t pad_rows)
 {
     __shared__ int32_t s[NBUCKET];
+    __shared__ int32_t ktot[ROW_MAXK];
     const int tid = threadIdx.x;
-    const int idx = tid < 128 ? (asc ? 128 + tid : 255 - tid) : (asc ? tid - 128 : 255 - tid);
-    const int32_t v = hist[idx];
+    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NKB = B_MID / ROW_MAXK;
+    int idx;
+    if (tid < NW) idx = asc ? B_WIDE + tid : NBUCKET - 1 - tid;
+    else if (tid < NW + NM) idx = asc ? B_MID + (tid - NW) : B_WIDE - 1 - (tid - NW);
+    else idx = NBUCKET - 1 - tid;
+    const int32_t count = hist[idx];
+    if (tid < ROW_MAXK) ktot[tid] = 0;
+    __syncthreads();
+    if (idx < B_MID && count) atomicAdd(&ktot[idx / NKB], count);
+    __syncthreads();
+    // the lowest bucket of each K part comes last in the scan and takes the pad
+    const int pad = (pad_rows && idx < B_MID && idx % NKB == 0) ? (ROWS - ktot[idx / NKB] % ROWS) % ROWS : 0;
+    const int32_t v = count + pad;
     s[tid] = v;
     __syncthreads();
     for (int off = 1; off < NBUCKET; off <<= 1) {
@@ -192,7 +250,10 @@ __global__ void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ o
         __syncthreads();
     }
     offs[idx] = s[tid] - v;
-    if (tid == 127) tally[T_NLARGE] = s[tid];
+    for (int j = 0; j < pad; ++j) order[s[tid] - pad + j] = -1;
+    if (pad) atomicAdd(&tally[T_NPAD], pad);
+    if (tid == NW - 1) tally[T_NLARGE] = s[tid];
+    if (tid == NW + NM - 1) tally[T_NMID] = s[tid];
     if (tid == NBUCKET - 1) tally[T_NTASK] = s[tid];
 }
 
@@ -243,8 +304,9 @@ __device__ __forceinline__ double price_option(const OptConst &cv, const double 
     const int n = uni(cv.n), e = uni(cv.e);
     load_edge_table(edge, lane, etab);
     int flo, fhi, blo, bhi;
-    forward_pass<T, KMAX>(t, e, lane, lds, etab, fscratch, flo, fhi);
-    backward_pass<T, KMAX>(t, n, e, lane, lds, etab, blo, bhi);
+    const bool unit = uni((int)cv.unit) != 0;
+    forward_pass<T, KMAX>(t, e, lane, lds, etab, fscratch, flo, fhi, unit);
+    backward_pass<T, KMAX>(t, n, e, lane, lds, etab, blo, bhi, unit);
 
     const int ep = min(e, t.jmax);
     const int klo = t.jmax - ep, khi = t.jmax + ep;
@@ -380,13 +442,63 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
 }
 
+// Row-class tasks [first, last) in groups of four consecutive tasks, one
+// group per wave at a time (static stride); the last group may hold fewer
+// options, and its idle rows repeat the group's first option without
+// writing a price.
+template <class T>
+__device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                                                 const int32_t *__restrict__ order, int first, int last,
+                                                 double *__restrict__ out)
+{
+    __shared__ double rowbuf[2 * ROWS * ROW_MAX_WIDTH];
+    const int lane = threadIdx.x, row = lane / ROW, l = lane % ROW;
+    lds_f64 *ybuf = (lds_f64 *)rowbuf + row * ROW_MAX_WIDTH;
+    lds_f64 *bbuf = (lds_f64 *)rowbuf + (ROWS + row) * ROW_MAX_WIDTH;
+    const int ngroups = (last - first + ROWS - 1) / ROWS;
+    for (int g = uni((int)blockIdx.x); g < ngroups; g += uni((int)gridDim.x)) {
+        const int base = first + g * ROWS;
+        const int mine = base + row < last ? order[base + row] : -1;
+        const bool active = mine >= 0;
+        const int o = active ? mine : order[base];  // a group's first slot is never idle
+        const OptConst c = cst[o];
+        RowOpt r;
+        r.t = TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
+        r.n = c.n;
+        r.e = c.e;
+        r.type = c.type;
+        r.X = c.X;
+        r.Pe = c.Pe;
+        r.Pn = c.Pn;
+        r.edge = edge + (size_t)o * EDGE_TAB;
+        const int e_row = active ? c.e : 0, n_row = active ? c.n : 0;
+        int ev[8];
+        int K = 1;
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            ev[q] = __builtin_amdgcn_readlane(e_row, q * ROW);
+            ev[ROWS + q] = __builtin_amdgcn_readlane(n_row, q * ROW);
+            K = max(K, (__builtin_amdgcn_readlane(c.w, q * ROW) + ROW - 1) / ROW);
+        }
+        double p;
+        switch (K) {
+        case 1: p = price_rows<T, 1>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
+        case 2: p = price_rows<T, 2>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
+        case 3: p = price_rows<T, 3>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
+        default: p = price_rows<T, 4>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
+        }
+        if (active && l == 0) out[o] = p;
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                                        const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                                        double *__restrict__ out)
 {
-    const int nlarge = uni(tally[T_NLARGE]), ntask = uni(tally[T_NTASK]);
-    price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, ntask, out, nullptr);
+    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
+    price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr);
+    price_row_groups<T>(cst, edge, order, nmid, ntask, out);
 }
 
 // Form 1: one lane per option over every scheduled task (tasks ordered by
@@ -398,6 +510,7 @@ __global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__r
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= tally[T_NTASK]) return;
     const int o = order[gid];
+    if (o < 0) return;  // idle pad slot
     const OptConst c = cst[o];
     double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)LANE_MAX_WIDTH * WAVE) + (gid % WAVE);
     out[o] = lane_price(c, edge + (size_t)o * EDGE_TAB, LaneColumn{base},
@@ -547,8 +660,11 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
-        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, p->cst, p->edge, p->bucket, p->hist, p->tally, out);
-        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, asc);
+        const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
+        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, unit_dmin, p->cst, p->edge, p->bucket,
+                                               p->hist, p->tally, out);
+        // the lane form prices tasks one per lane: no row groups, no pads
+        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, p->order, asc, p->cfg.form != HWT_FORM_LANE);
         k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
@@ -676,7 +792,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_cst = carve(cap * sizeof(OptConst));
     const size_t o_edge = carve(cap * EDGE_TAB * sizeof(double));
     const size_t o_bucket = carve(cap * sizeof(int32_t));
-    const size_t o_order = carve(cap * sizeof(int32_t));
+    const size_t o_order = carve((cap + ROWS * ROW_MAXK) * sizeof(int32_t));  // + row-class pads
     const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
     const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
@@ -835,7 +951,7 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
         stats->node_updates = w;
         stats->wide_node_updates = ww;
         stats->seq_node_updates = sw;
-        stats->tasks = t[T_NTASK];
+        stats->tasks = t[T_NTASK] - t[T_NPAD];
         stats->wide_tasks = t[T_NLARGE];
         stats->unsupported = t[T_UNSUP];
     }

@@ -222,3 +222,22 @@ def test_plan_capacity_enforced(pkg):
     assert plan.execute_status(dev, n=10) == pkg.capi.HWT_OK
     plan.sync()
     plan.close()
+
+
+@pytest.mark.parametrize("curve", ["unsorted", "long", "single"])
+def test_yield_curve_shapes(pkg, oracle, curve):
+    """The setup kernel reads curves that fit in LDS by binary search when
+    their days never decrease, and keeps the reference's linear scan
+    (common/Domain.hpp:11-46) otherwise: every shape prices as the oracle."""
+    base = pkg.datasets.synthetic_yield()
+    if curve == "unsorted":
+        perm = np.arange(base.N)
+        perm[[10, 11, 60, 61]] = perm[[11, 10, 61, 60]]
+        y = pkg.formats.Yield(base.rates[perm], base.days[perm])
+    elif curve == "long":  # more knots than the setup kernel stages
+        k = np.arange(1, 3001)
+        y = pkg.formats.Yield(0.01 + 0.03 * (k - 1) / 2999.0, np.rint(365.0 * 40 * k / 3000).astype(np.int32))
+    else:
+        y = pkg.formats.Yield(np.array([0.025]), np.array([365], np.int32))
+    opts = pkg.Options.concat([pkg.datasets.book(), pkg.datasets.skewed(seed=9, n=200)])
+    assert_parity(pkg.capi.price(opts, y), oracle.price(opts, y), curve)
