@@ -125,7 +125,8 @@ SOpts sopts(const hwt_options_f64 &o)
 // B~ = dt (1 - q^m) / (1 - q), q = e^{-a dt}, m = n - e (discrete bond
 // sensitivity); c0 = ln(100 P(S)/P(T)) minus the continuous-time convexity
 // terms B sigma^2 (1-e^{-aT})^2 / (2a^2) + B^2 sigma^2 (1 - e^{-2aT}) / (4a).
-__device__ bool surr_inputs(const SOpts &o, const SYield &y, int i, float x[NIN], double &base)
+template <class YieldFn>
+__device__ bool surr_inputs(const SOpts &o, int i, float x[NIN], double &base, const YieldFn &yield)
 {
     const double X = o.strike[i], Tb = o.maturity[i], L = o.length[i], a = o.a[i], sg = o.sigma[i];
     const int tu = o.tu[i], st = o.steps[i], ty = o.type[i];
@@ -138,8 +139,8 @@ __device__ bool surr_inputs(const SOpts &o, const SYield &y, int i, float x[NIN]
         c.n <= c.e)
         return false;
     const double T = c.e * c.dt, S = c.n * c.dt;
-    const double PT = exp(-yield_at(T, tu, y.rate, y.day, y.n) * T);
-    const double PS = exp(-yield_at(S, tu, y.rate, y.day, y.n) * S);
+    const double PT = exp(-yield(T, tu) * T);
+    const double PS = exp(-yield(S, tu) * S);
     const double Bc = (1.0 - exp(-a * (S - T))) / a;
     const double sp = sg * Bc * sqrt((1.0 - exp(-2.0 * a * T)) / (2.0 * a));
     const double h = log(100.0 * PS / (X * PT)) / sp + 0.5 * sp;
@@ -173,7 +174,8 @@ __global__ void k_train_inputs(SOpts o, SYield y, const double *__restrict__ pri
     if (i >= o.n) return;
     float x[NIN];
     double base = 0.0;
-    const bool ok = surr_inputs(o, y, i, x, base);
+    const bool ok = surr_inputs(o, i, x, base,
+                                [&](double t, int tu) { return yield_at(t, tu, y.rate, y.day, y.n); });
 #pragma unroll
     for (int k = 0; k < NIN; ++k) X[(size_t)i * NIN + k] = ok ? x[k] : 0.0f;
     target[i] = ok ? (float)((price[i] - base) / RES) : 0.0f;
@@ -511,10 +513,20 @@ __global__ void k_refresh_bf16(Net net, int members)
 }
 
 // --------------------------------------------------------------- inference
-// One workgroup prices INF_ROWS options with every member.  LDS holds the
-// inputs and one BF16 activation tile; wave w owns rows [32w, 32w + 32) of the
-// tile and rewrites only those, so the layers chain without barriers between
-// the waves' GEMMs.  The hidden layers read their weights straight from L2.
+// One workgroup (4 waves) prices INF_ROWS = 128 options with every member.
+// LDS holds the inputs and the BF16 activation tile act[row][neuron].  The
+// hidden layers run as D = W A^T on the matrix cores: the weights are the
+// A operand (loaded from L2, K-contiguous rows of W), the activations the B
+// operand (LDS rows), so D[neuron][row] lands with four consecutive neurons
+// of one row in each lane's accumulator quad and the epilogue writes them as
+// one 8-byte store.  Wave w owns neurons [64w, 64w + 64) for all 128 rows,
+// so each weight is read once per tile; a layer reads the whole tile, a
+// barrier, then the epilogue overwrites it in place.  Layer 1 (FP32, the
+// training kernel's order) reads its wave-uniform weights through the scalar
+// cache; the output layer reduces per row across lanes and waves in a fixed
+// order.  The yield curve is staged in LDS for the inputs.
+constexpr int INF_YMAX = 256;  // keeps two 128-row tiles per CU in LDS
+
 struct InferArgs {
     SOpts o;
     SYield y;
@@ -529,22 +541,43 @@ struct InferArgs {
     int32_t *fb_count; // [0] total, [1] out of box
 };
 
-__global__ __launch_bounds__(256) void k_infer(InferArgs g)
+__device__ inline uint32_t pack_bf16x2(float a, float b) { return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_infer(InferArgs g)
 {
+    static_assert(INF_ROWS == 128, "the layer-1 and head mappings assume 128 rows per tile");
     __shared__ __attribute__((aligned(16))) uint16_t act[INF_ROWS * ACT_LD];
     __shared__ float xs[INF_ROWS * NIN];
     __shared__ double bases[INF_ROWS];
+    __shared__ float hp[4 * INF_ROWS];
     __shared__ float ysum[INF_ROWS], ymin[INF_ROWS], ymax[INF_ROWS];
     __shared__ int32_t okf[INF_ROWS];
     __shared__ int32_t fb_local[INF_ROWS];
     __shared__ int32_t fb_n, fb_out, fb_base;
+    __shared__ double yr[INF_YMAX];
+    __shared__ int32_t yd[INF_YMAX];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r0 = blockIdx.x * INF_ROWS;
+
+    // curves that fit are read from LDS, by binary search when their days
+    // never decrease; both lookups give the reference's value
+    const bool staged = g.y.n <= INF_YMAX;
+    int unsorted = 0;
+    if (staged) {
+        for (int k = t; k < g.y.n; k += 256) {
+            yr[k] = g.y.rate[k];
+            yd[k] = g.y.day[k];
+            if (k + 1 < g.y.n && g.y.day[k] > g.y.day[k + 1]) unsorted = 1;
+        }
+    }
+    const bool sorted = staged && !__syncthreads_or(unsorted);
     if (t < INF_ROWS) {
         const int i = r0 + t;
         float x[NIN] = {};
         double base = 0.0;
-        const bool ok = i < g.o.n && surr_inputs(g.o, g.y, i, x, base);
+        const bool ok = i < g.o.n && surr_inputs(g.o, i, x, base, [&](double tt, int tu) {
+            return sorted ? yield_at_sorted(tt, tu, yr, yd, g.y.n) : yield_at(tt, tu, g.y.rate, g.y.day, g.y.n);
+        });
 #pragma unroll
         for (int k = 0; k < NIN; ++k) xs[t * NIN + k] = x[k];
         bases[t] = base;
@@ -555,82 +588,104 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g)
     }
     if (t == 0) fb_n = fb_out = 0;
     __syncthreads();
+
+    const int mrow = lane & 31, khalf = 8 * (lane >> 5);
+    const int n0 = wave * 64;  // this wave's output neurons in the hidden layers
     for (int m = 0; m < g.members; ++m) {
         const float *P = g.net.P + (size_t)m * PSTRIDE;
-        // input layer: thread t = neuron t
+        // layer 1: thread t -> row t % 128, neurons [128 (t / 128), +128)
         {
-            float w[NIN];
+            const int r = t & (INF_ROWS - 1), c0 = (t >> 7) * 128;
+            float x[NIN];
 #pragma unroll
-            for (int k = 0; k < NIN; ++k) w[k] = P[HWT_SURR_W1 + t * NIN + k];
-            const float b = P[HWT_SURR_B1 + t];
-            for (int r = 0; r < INF_ROWS; ++r) {
-                float acc = b;
+            for (int k = 0; k < NIN; ++k) x[k] = xs[r * NIN + k];
+            for (int c = c0; c < c0 + 128; c += 4) {
+                float v[4];
 #pragma unroll
-                for (int k = 0; k < NIN; ++k) acc = fmaf(w[k], xs[r * NIN + k], acc);
-                act[r * ACT_LD + t] = to_bf16(silu(acc));
+                for (int j = 0; j < 4; ++j) {
+                    float acc = P[HWT_SURR_B1 + c + j];
+#pragma unroll
+                    for (int k = 0; k < NIN; ++k) acc = fmaf(P[HWT_SURR_W1 + (c + j) * NIN + k], x[k], acc);
+                    v[j] = silu(acc);
+                }
+                *reinterpret_cast<uint2 *>(act + r * ACT_LD + c) = make_uint2(pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]));
             }
         }
         __syncthreads();
-        const int rw = wave * 32;  // this wave's rows
         for (int l = 2; l <= 4; ++l) {
             const uint16_t *W = g.net.Wb + ((size_t)m * 3 + (l - 2)) * LAYER_W;
-            f32x16 acc[8];
+            const uint16_t *wa0 = W + (size_t)(n0 + mrow) * HID + khalf;
+            const uint16_t *wa1 = wa0 + (size_t)32 * HID;
+            const uint16_t *ab = act + mrow * ACT_LD + khalf;
+            f32x16 acc[2][4];
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x16{};
-            const uint16_t *arow = act + (rw + (lane & 31)) * ACT_LD + 8 * (lane >> 5);
-            const uint16_t *wrow = W + (size_t)(lane & 31) * HID + 8 * (lane >> 5);
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x16{};
+#pragma unroll 2
             for (int k = 0; k < HID; k += 16) {
-                const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(arow + k));
+                const bf16x8 a0 = load_frag(wa0 + k), a1 = load_frag(wa1 + k);
 #pragma unroll
-                for (int cb = 0; cb < 8; ++cb) {
-                    const bf16x8 b = load_frag(wrow + (size_t)cb * 32 * HID + k);
-                    acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[cb], 0, 0, 0);
+                for (int nb = 0; nb < 4; ++nb) {
+                    const bf16x8 b =
+                        __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(ab + nb * 32 * ACT_LD + k));
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][nb], 0, 0, 0);
                 }
             }
+            __syncthreads();  // every wave has read this layer's input tile
             const float *bias = P + boff(l);
             if (l < 4) {
 #pragma unroll
-                for (int cb = 0; cb < 8; ++cb) {
-                    const int col = cb * 32 + (lane & 31);
-                    const float bc = bias[col];
+                for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        const int row = rw + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                        act[row * ACT_LD + col] = to_bf16(silu(acc[cb][v] + bc));
+                    for (int q = 0; q < 4; ++q) {
+                        const int col = n0 + 32 * mb + 8 * q + 4 * (lane >> 5);
+                        const float4 bv = *reinterpret_cast<const float4 *>(bias + col);
+#pragma unroll
+                        for (int nb = 0; nb < 4; ++nb) {
+                            const f32x16 &a = acc[mb][nb];
+                            const uint2 pk = make_uint2(pack_bf16x2(silu(a[4 * q] + bv.x), silu(a[4 * q + 1] + bv.y)),
+                                                        pack_bf16x2(silu(a[4 * q + 2] + bv.z), silu(a[4 * q + 3] + bv.w)));
+                            *reinterpret_cast<uint2 *>(act + (32 * nb + mrow) * ACT_LD + col) = pk;
+                        }
                     }
-                }
+                __syncthreads();
             } else {
-                // output layer in FP32 straight from the accumulators
-                float part[16];
+                // output layer in FP32 from the accumulators: per row, the
+                // lane's 32 neurons, then the lane pair holding the same row,
+                // then the four waves
+                float p[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int v = 0; v < 16; ++v) part[v] = 0.0f;
+                for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-                for (int cb = 0; cb < 8; ++cb) {
-                    const int col = cb * 32 + (lane & 31);
-                    const float bc = bias[col], w5 = P[HWT_SURR_W5 + col];
+                    for (int q = 0; q < 4; ++q) {
+                        const int col = n0 + 32 * mb + 8 * q + 4 * (lane >> 5);
+                        const float4 bv = *reinterpret_cast<const float4 *>(bias + col);
+                        const float4 w5 = *reinterpret_cast<const float4 *>(P + HWT_SURR_W5 + col);
 #pragma unroll
-                    for (int v = 0; v < 16; ++v) part[v] = fmaf(w5, silu(acc[cb][v] + bc), part[v]);
-                }
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    float p = part[v];
-#pragma unroll
-                    for (int o = 16; o >= 1; o >>= 1) p += __shfl_xor(p, o, 64);
-                    part[v] = p;
-                }
-                if ((lane & 31) == 0) {
-                    const float b5 = P[HWT_SURR_B5];
-#pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        const int row = rw + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                        const float yv = part[v] + b5;
-                        ysum[row] += yv;
-                        ymin[row] = fminf(ymin[row], yv);
-                        ymax[row] = fmaxf(ymax[row], yv);
+                        for (int nb = 0; nb < 4; ++nb) {
+                            const f32x16 &a = acc[mb][nb];
+                            p[nb] = fmaf(w5.x, silu(a[4 * q] + bv.x), p[nb]);
+                            p[nb] = fmaf(w5.y, silu(a[4 * q + 1] + bv.y), p[nb]);
+                            p[nb] = fmaf(w5.z, silu(a[4 * q + 2] + bv.z), p[nb]);
+                            p[nb] = fmaf(w5.w, silu(a[4 * q + 3] + bv.w), p[nb]);
+                        }
                     }
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const float v = p[nb] + __shfl_xor(p[nb], 32, 64);
+                    if (lane < 32) hp[wave * INF_ROWS + 32 * nb + lane] = v;
+                }
+                __syncthreads();
+                if (t < INF_ROWS) {
+                    const float yv = ((hp[t] + hp[INF_ROWS + t]) + (hp[2 * INF_ROWS + t] + hp[3 * INF_ROWS + t])) +
+                                     P[HWT_SURR_B5];
+                    ysum[t] += yv;
+                    ymin[t] = fminf(ymin[t], yv);
+                    ymax[t] = fmaxf(ymax[t], yv);
                 }
             }
-            __syncthreads();  // this layer's activations are complete before the next reads them
         }
     }
     if (t < INF_ROWS) {

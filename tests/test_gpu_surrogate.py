@@ -9,6 +9,8 @@ Bars:
 - gradients: per tensor, ||g_gpu - g_cpu|| <= 1e-2 ||g_cpu|| (the GPU also
   rounds back-propagated values to BF16); a transposed or misplaced operand
   gives errors of order one;
+- inference kernel: ensemble mean and spread within 0.01 * 4e-4 * max(1, |y|)
+  of the CPU members' BF16-emulating forward (other accumulation order);
 - Adam: the closed-form first and second steps to FP32 rounding;
 - gate: with threshold 0 every option is re-priced by the tree and the output
   equals the tree's bit for bit; options outside the box always go to the tree.
@@ -189,3 +191,24 @@ def test_checkpoint_resume_is_exact(pkg, setup, tmp_path):
     np.testing.assert_array_equal(a.get_params(), b.get_params())
     a.close()
     b.close()
+
+
+def test_inference_kernel_matches_cpu_reference(setup):
+    """The fused inference kernel (all members, ensemble mean and spread)
+    against the BF16-emulating CPU forward of each member, on training rows:
+    mean = closed form + 0.01 * mean of the member outputs, spread = 0.01 *
+    (max - min).  The closed form is recovered from the stored residual
+    target (FP32), so the bar carries its rounding too."""
+    s, tree = setup["s"], setup["tree"]
+    P = s.get_params().astype(np.float64)
+    mean, spread, inbox = s.predict(setup["dev"])
+    rows = np.arange(3, NTRAIN, 257)
+    x, t = s.get_inputs(rows)
+    ys = np.stack([R.forward_bf16(P[m], x)[0] for m in range(s.members)])
+    base = tree[rows] - t.astype(np.float64) * R.RES
+    want_mean = base + R.RES * ys.mean(axis=0)
+    want_spread = R.RES * (ys.max(axis=0) - ys.min(axis=0))
+    assert inbox[rows].all()
+    tol = R.RES * 4e-4 * np.maximum(1.0, np.abs(ys).max(axis=0)) + 1e-6 * np.abs(t) * R.RES + 1e-12
+    np.testing.assert_array_less(np.abs(mean[rows] - want_mean), tol)
+    np.testing.assert_array_less(np.abs(spread[rows] - want_spread), 2 * tol)
