@@ -80,11 +80,16 @@ def surrogate_leg(pkg, y, args):
     S = pkg.surrogate
     s, fit = S.fit(y, n_train=2_000_000, epochs=args.surrogate_epochs, seed=11,
                    log=lambda m: print(m, file=sys.stderr, flush=True))
-    ev = S.evaluate(s, y, n=1_000_000, seed=5, reps=max(1, min(args.steps, 10)))
+    # the gate's spread threshold, calibrated on a held-out sample (its own
+    # seed) so that surrogate-priced options stay within 0.7e-3 of the tree
+    cal = S.calibrate_threshold(s, y, n=250_000, seed=77, target=1e-3, margin=0.7)
+    gated = S.with_threshold(s, cal["threshold"], 1 << 20, y.N)
+    ev = S.evaluate(gated, y, n=1_000_000, seed=5, reps=max(1, min(args.steps, 10)))
+    gated.close()
     s.close()
     return {"workload": "1,000,000 options: 95% inside the training box, 5% outside (a in [0.2,0.4] "
                         "or expiry 30-40y); ensemble BF16 MLP inference + FP64 tree fallback",
-            "training": fit, **ev}
+            "training": fit, "calibration": cal, **ev}
 
 
 def main():
@@ -152,14 +157,20 @@ def main():
         plan.close()
         return e0.elapsed_ms(e1) / args.steps, stats, dev["out"].to_host()
 
-    ms_per_step, st, fp64_prices = time_graph(capi.config())
+    # the project autotuner picks the form-2 settings for this batch first
+    tuned, tune_rep = capi.autotune(dev)
+    tuned_mixed, tune_rep_mixed = capi.autotune(dev, capi.config(precision=capi.PREC_MIXED))
+    ms_per_step, st, fp64_prices = time_graph(tuned)
     # mixed precision on the same batch: FP32 tree passes, FP64 discounting and sums
-    mixed_ms, _, mixed_prices = time_graph(capi.config(precision=capi.PREC_MIXED))
+    mixed_ms, _, mixed_prices = time_graph(tuned_mixed)
     mixed_err = np.abs(mixed_prices - fp64_prices)
 
     # dominant kernel on its own: the same pipeline launched without the graph,
     # HIP events around the wide-tree kernel on the stream it runs on
-    plan_k = capi.Plan(opts.N, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+    cfg_k = capi.config()
+    C.memmove(C.byref(cfg_k), C.byref(tuned), C.sizeof(cfg_k))
+    cfg_k.flags |= capi.FLAG_NO_GRAPH
+    plan_k = capi.Plan(opts.N, y.N, cfg_k)
     wide, narrow, kern = [], [], []
     for i in range(args.warmup + args.steps):
         plan_k.execute(dev)
@@ -206,6 +217,7 @@ def main():
             "options": opts.N,
             "precision": "fp64",
             "pipeline": "device constants + LPT ordering + wide/narrow persistent pricing kernels, one HIP graph",
+            "tuned": tuned.as_dict(),
         },
         "roofline": {
             "bound": "valu",
@@ -230,6 +242,7 @@ def main():
                                       "note": "hwt_price_f64 on host arrays: plan creation, H2D copies, "
                                               "pricing, D2H copy and plan destruction (wall clock)"},
             "graph_total_ms_per_step": ms_per_step,
+            "autotune": {"fp64": tune_rep, "mixed": tune_rep_mixed, "mixed_config": tuned_mixed.as_dict()},
             "mixed_precision": {
                 "options_per_s": opts.N / (mixed_ms * 1e-3),
                 "ms_per_step": mixed_ms,

@@ -19,12 +19,12 @@ LIB_PATH = PKG_DIR / "libhwtree.so"
 HWT_OK, HWT_EINVAL, HWT_EUNSUPPORTED, HWT_ENOMEM, HWT_EHIP = 0, -1, -2, -3, -4
 FORM_AUTO, FORM_LANE, FORM_WAVE = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
-FLAG_NO_GRAPH = 1
+FLAG_NO_GRAPH, FLAG_NO_ROWS, FLAG_SERIAL = 1, 2, 4
 
 # The exported C symbols, in include/hwtree.h order.
 EXPORTS = (
     "hwt_config_init", "hwt_price_f64", "hwt_plan_create", "hwt_plan_execute",
-    "hwt_plan_sync", "hwt_plan_destroy", "hwt_alphas_f64", "hwt_max_width",
+    "hwt_plan_sync", "hwt_plan_destroy", "hwt_alphas_f64", "hwt_autotune", "hwt_max_width",
     "hwt_strerror", "hwt_last_error", "hwt_abi_version",
 )
 
@@ -49,7 +49,20 @@ class YieldC(C.Structure):
 class ConfigC(C.Structure):
     _fields_ = [("form", C.c_int32), ("precision", C.c_int32), ("block_size", C.c_int32),
                 ("sort", C.c_int32), ("device", C.c_int32), ("flags", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("wide_per_cu", C.c_int32), ("narrow_per_cu", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["sort"] = chr(d["sort"]) if d["sort"] else "L"
+        return d
+
+
+class TuneReportC(C.Structure):
+    _fields_ = [("ms", C.c_double), ("default_ms", C.c_double), ("tune_ms", C.c_double),
+                ("candidates", C.c_int32), ("cached", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class StatsC(C.Structure):
@@ -81,6 +94,8 @@ def lib():
         L.hwt_plan_destroy.argtypes = [P]
         L.hwt_plan_destroy.restype = None
         L.hwt_alphas_f64.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, P]
+        L.hwt_autotune.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, C.POINTER(ConfigC),
+                                   C.POINTER(TuneReportC)]
         L.hwt_max_width.argtypes = [C.c_int32]
         L.hwt_max_width.restype = C.c_int32
         L.hwt_strerror.argtypes = [C.c_int]
@@ -182,6 +197,17 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def autotune(dev: dict, cfg: ConfigC | None = None):
+    """hwt_autotune on a device batch (see ``to_device``): (tuned config, report)."""
+    c = cfg if cfg is not None else config()
+    n = int(dev["n"])
+    o = OptionsC(n, *[dev[f].value for f in Plan.FIELDS])
+    y = YieldC(int(dev["ny"]), dev["rates"].value, dev["days"].value)
+    rep = TuneReportC()
+    _check(lib().hwt_autotune(C.byref(o), C.byref(y), dev["out"].value, C.byref(c), C.byref(rep)))
+    return c, rep.as_dict()
 
 
 def to_device(opts: Options, y: Yield) -> dict:

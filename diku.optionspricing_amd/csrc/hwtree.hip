@@ -38,6 +38,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <chrono>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -119,7 +124,7 @@ constexpr int SETUP_BLOCK = 256;
 constexpr int SETUP_YMAX = 2048;  // knots staged in LDS by k_setup (24 KB)
 
 __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
-                                                       double unit_dmin,
+                                                       double unit_dmin, int32_t rows,
                                                        OptConst *__restrict__ cst, double *__restrict__ edge,
                                                        int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
                                                        int32_t *__restrict__ tally, double *__restrict__ out)
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 }
                 const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
                 if (wide) b = B_WIDE + log_bucket(key, 8, NBUCKET - B_WIDE);
-                else if (c.w > ROW_MAX_WIDTH) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
+                else if (c.w > ROW_MAX_WIDTH || !rows) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
                 else b = row_bucket(c.w, c.n);
                 atomicAdd(&lh[b], 1);
                 if (wide) atomicAdd(&lsum[2], (unsigned long long)c.work);
@@ -661,10 +666,11 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
         const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
-        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, unit_dmin, p->cst, p->edge, p->bucket,
-                                               p->hist, p->tally, out);
+        const bool rows = p->cfg.form != HWT_FORM_LANE && !(p->cfg.flags & HWT_FLAG_NO_ROWS);
+        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, unit_dmin, rows, p->cst, p->edge,
+                                               p->bucket, p->hist, p->tally, out);
         // the lane form prices tasks one per lane: no row groups, no pads
-        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, p->order, asc, p->cfg.form != HWT_FORM_LANE);
+        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, p->order, asc, rows);
         k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
@@ -674,22 +680,30 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
             HWT_HIP(hipGetLastError());
             return HWT_OK;
         }
-        HWT_HIP(hipEventRecord(p->fork, s));
-        HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, p->side));
-        if (p->cfg.precision == HWT_PREC_MIXED)
-            k_price_narrow<float><<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
-        else
-            k_price_narrow<double><<<p->grid_narrow, WAVE, 0, p->side>>>(p->cst, p->edge, p->order, p->tally, out);
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, p->side));
+        // the narrow-tree kernel beside the wide-tree one on the side stream,
+        // or after it on `s` (HWT_FLAG_SERIAL)
+        const bool serial = (p->cfg.flags & HWT_FLAG_SERIAL) != 0;
+        hipStream_t ns = serial ? s : p->side;
+        if (!serial) {
+            HWT_HIP(hipEventRecord(p->fork, s));
+            HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+        }
         if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
         if (p->cfg.precision == HWT_PREC_MIXED)
             k_price_wide<float><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
         else
             k_price_wide<double><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evw1, s));
-        HWT_HIP(hipEventRecord(p->join, p->side));
-        HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, ns));
+        if (p->cfg.precision == HWT_PREC_MIXED)
+            k_price_narrow<float><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out);
+        else
+            k_price_narrow<double><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out);
+        if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, ns));
+        if (!serial) {
+            HWT_HIP(hipEventRecord(p->join, p->side));
+            HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
+        }
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
     }
     HWT_HIP(hipGetLastError());
@@ -740,7 +754,8 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     if (cfg) p->cfg = *cfg;
     else hwt_config_init(&p->cfg);
     if (p->cfg.form < HWT_FORM_AUTO || p->cfg.form > HWT_FORM_WAVE ||
-        (p->cfg.precision != HWT_PREC_FP64 && p->cfg.precision != HWT_PREC_MIXED)) {
+        (p->cfg.precision != HWT_PREC_FP64 && p->cfg.precision != HWT_PREC_MIXED) || p->cfg.wide_per_cu < 0 ||
+        p->cfg.narrow_per_cu < 0) {
         delete p;
         return fail(HWT_EINVAL, "hwt_plan_create: unknown form or precision");
     }
@@ -777,6 +792,8 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
             hipSuccess ||
         per_cu_n < 1)
         per_cu_n = 8;
+    if (p->cfg.wide_per_cu > 0) per_cu_w = std::min(per_cu_w, p->cfg.wide_per_cu);
+    if (p->cfg.narrow_per_cu > 0) per_cu_n = std::min(per_cu_n, p->cfg.narrow_per_cu);
     p->grid_wide = per_cu_w * cus;
     p->grid_narrow = per_cu_n * cus;
     if (const char *v = getenv("HWT_VERBOSE"); v && *v && *v != '0')
@@ -1099,3 +1116,172 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, int6
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------- autotuner
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, hwt_config> g_tune_cache;
+bool g_tune_loaded = false;
+
+int ilog2(int64_t v)
+{
+    int b = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+std::string tune_signature(const hwt_config &c, int n, const hwt_stats &st)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::ostringstream os;
+    os << "d" << dev << ".p" << c.precision << ".n" << ilog2(n) << ".t" << ilog2(st.tasks) << ".w"
+       << ilog2(st.wide_tasks) << ".u" << ilog2(st.node_updates);
+    return os.str();
+}
+
+void tune_load_file()
+{
+    const char *path = getenv("HWT_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::ifstream f(path);
+    std::string key;
+    hwt_config c;
+    hwt_config_init(&c);
+    while (f >> key >> c.sort >> c.flags >> c.wide_per_cu >> c.narrow_per_cu) g_tune_cache[key] = c;
+}
+
+void tune_store_file(const std::string &key, const hwt_config &c)
+{
+    const char *path = getenv("HWT_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::ofstream f(path, std::ios::app);
+    f << key << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL)) << ' ' << c.wide_per_cu << ' '
+      << c.narrow_per_cu << '\n';
+}
+
+// Median pipeline time of three timed executions after one warm-up.
+int tune_time(const hwt_options_f64 *o, const hwt_yield_f64 *y, double *out, hwt_config c, double &ms,
+              hwt_stats *first)
+{
+    c.flags |= HWT_FLAG_NO_GRAPH;
+    hwt_plan *p = nullptr;
+    int rc = hwt_plan_create(&p, o->n, y->n, &c);
+    if (rc) return rc;
+    double t[3];
+    for (int i = 0; i < 4 && rc == HWT_OK; ++i) {
+        hwt_stats st{};
+        rc = hwt_plan_execute(p, o, y, out, nullptr);
+        if (rc == HWT_OK) rc = hwt_plan_sync(p, &st);
+        if (rc == HWT_EUNSUPPORTED) rc = HWT_OK;  // unsupported options are NaN in every candidate alike
+        if (i == 0 && first) *first = st;
+        if (i > 0) t[i - 1] = st.total_ms;
+    }
+    hwt_plan_destroy(p);
+    if (rc) return rc;
+    std::sort(t, t + 3);
+    ms = t[1];
+    return HWT_OK;
+}
+
+} // namespace
+
+extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64 *dev_yield, double *dev_out,
+                            hwt_config *cfg, hwt_tune_report *report)
+{
+    g_last_error.clear();
+    if (!cfg || !valid_opts_ptrs(dev_opts) || !valid_yield_ptrs(dev_yield) || (dev_opts->n > 0 && !dev_out))
+        return fail(HWT_EINVAL, "hwt_autotune: null or negative-size argument");
+    const auto w0 = std::chrono::steady_clock::now();
+    hwt_tune_report rep{};
+    if (cfg->form == HWT_FORM_LANE || dev_opts->n == 0) {  // nothing of form 2 to choose
+        if (report) *report = rep;
+        return HWT_OK;
+    }
+    const hwt_config base = *cfg;
+    hwt_stats st{};
+    double base_ms = 0.0;
+    int rc = tune_time(dev_opts, dev_yield, dev_out, base, base_ms, &st);
+    if (rc) return rc;
+    rep.default_ms = base_ms;
+    const std::string key = tune_signature(base, dev_opts->n, st);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (!g_tune_loaded) {
+            tune_load_file();
+            g_tune_loaded = true;
+        }
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            hwt_config c = base;
+            c.sort = it->second.sort;
+            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL)) |
+                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL));
+            c.wide_per_cu = it->second.wide_per_cu;
+            c.narrow_per_cu = it->second.narrow_per_cu;
+            *cfg = c;
+            rep.cached = 1;
+        }
+    }
+    if (rep.cached) {
+        rc = tune_time(dev_opts, dev_yield, dev_out, *cfg, rep.ms, nullptr);
+        if (rc) return rc;
+        rep.candidates = 1;
+        if (report) *report = rep;
+        return HWT_OK;
+    }
+    // greedy coordinate descent; a change must win by 2% to be kept
+    hwt_config best = base;
+    double best_ms = base_ms;
+    int cand = 1;
+    auto consider = [&](hwt_config c) -> int {
+        double ms = 0.0;
+        const int r = tune_time(dev_opts, dev_yield, dev_out, c, ms, nullptr);
+        if (r) return r;
+        ++cand;
+        if (ms < 0.98 * best_ms) {
+            best = c;
+            best_ms = ms;
+        }
+        return HWT_OK;
+    };
+    for (int so : {(int)HWT_SORT_WORK_DESC, (int)HWT_SORT_HEIGHT_DESC, (int)HWT_SORT_WIDTH_DESC}) {
+        if (so == (best.sort ? best.sort : HWT_SORT_WORK_DESC)) continue;
+        hwt_config c = best;
+        c.sort = so;
+        if ((rc = consider(c))) return rc;
+    }
+    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_SERIAL}) {
+        hwt_config c = best;
+        c.flags ^= bit;
+        if ((rc = consider(c))) return rc;
+    }
+    for (int v : {4, 2}) {
+        if (v == best.wide_per_cu) continue;
+        hwt_config c = best;
+        c.wide_per_cu = v;
+        if ((rc = consider(c))) return rc;
+    }
+    for (int v : {8, 4}) {
+        if (v == best.narrow_per_cu) continue;
+        hwt_config c = best;
+        c.narrow_per_cu = v;
+        if ((rc = consider(c))) return rc;
+    }
+    *cfg = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = best;
+        tune_store_file(key, best);
+    }
+    rep.ms = best_ms;
+    rep.candidates = cand;
+    rep.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    if (report) *report = rep;
+    return HWT_OK;
+}
+

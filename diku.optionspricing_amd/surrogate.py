@@ -226,6 +226,46 @@ def fit(y, n_train: int = 2_000_000, epochs: int = 40, seed: int = 11, infer_cap
     return s, rec
 
 
+def calibrate_threshold(s: Surrogate, y, n: int = 250_000, seed: int = 77, target: float = 1e-3,
+                        margin: float = 0.7) -> dict:
+    """The largest ensemble-spread threshold at which every option the
+    surrogate would price in a held-out sample (box_mix, its own seed) stays
+    within margin * target of the FP64 tree.  In-box options are sorted by
+    spread; the threshold is the spread of the last option before the running
+    maximum error first exceeds the bound."""
+    from . import datasets
+    opts = datasets.box_mix(seed, n)
+    dev = capi.to_device(opts, y)
+    plan = capi.Plan(n, y.N)
+    plan.execute(dev)
+    plan.sync()
+    plan.close()
+    tree = dev["out"].to_host()
+    mean, spread, inbox = s.predict(dev)
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()
+    idx = np.nonzero(inbox)[0]
+    order = idx[np.argsort(spread[idx], kind="stable")]
+    run = np.maximum.accumulate(np.abs(mean[order] - tree[order]))
+    ok = np.nonzero(run <= margin * target)[0]
+    k = int(ok[-1]) if ok.size and ok[0] == 0 else -1
+    # options tied at the chosen spread must all pass
+    while k >= 0 and k + 1 < order.size and spread[order[k + 1]] == spread[order[k]]:
+        k -= 1
+    thr = float(spread[order[k]]) if k >= 0 else 0.0
+    return {"threshold": thr, "calibration_options": n, "calibration_seed": seed,
+            "accepted_frac": (k + 1) / n, "bound": margin * target}
+
+
+def with_threshold(s: Surrogate, threshold: float, infer_capacity: int, yield_capacity: int) -> Surrogate:
+    """A copy of the ensemble (same parameters) gating at another spread threshold."""
+    cfg = config(members=s.members, spread_threshold=threshold)
+    t = Surrogate(0, infer_capacity, yield_capacity, cfg)
+    t.set_params(s.get_params())
+    return t
+
+
 def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, log=None) -> dict:
     """Surrogate + tree fallback against the plain FP64 tree on the mixed
     batch (95% inside the box, 5% outside): options/s of each (HIP events),

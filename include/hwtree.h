@@ -84,17 +84,26 @@ typedef struct {
 typedef struct {
     int32_t form;        /* HWT_FORM_*                                  (0 = auto) */
     int32_t precision;   /* HWT_PREC_*                                  (0 = FP64) */
-    int32_t block_size;  /* form 1: lanes per workgroup (64..1024); form 2: waves per
-                            workgroup x 64.  <= 0 picks the tuned default. */
-    int32_t sort;        /* HWT_SORT_* (0 = work-descending) */
+    int32_t block_size;  /* form 1: lanes per workgroup (64..1024); <= 0 picks the
+                            default.  Form 2 runs one wavefront per workgroup. */
+    int32_t sort;        /* HWT_SORT_*: processing order of the wide and mid-width
+                            trees (0 = work-descending) */
     int32_t device;      /* HIP device ordinal for hwt_price_f64; < 0 = current */
     int32_t flags;       /* HWT_FLAG_* */
-    int32_t reserved[2];
+    int32_t wide_per_cu;   /* form 2: resident workgroups per CU of the persistent
+                              wide-tree kernel; 0 = as many as fit */
+    int32_t narrow_per_cu; /* form 2: the same for the narrow-tree kernel */
 } hwt_config;
 
 /* Launch the plan's kernels one by one instead of replaying its HIP graph;
  * hwt_stats.kernel_ms then times the pricing kernel alone. */
 #define HWT_FLAG_NO_GRAPH 1
+/* Form 2: price trees up to 64 nodes wide one per wavefront instead of four
+ * per wavefront (16-lane rows). */
+#define HWT_FLAG_NO_ROWS  2
+/* Form 2: run the narrow-tree kernel after the wide-tree kernel on one stream
+ * instead of beside it on a second stream. */
+#define HWT_FLAG_SERIAL   4
 
 typedef struct {
     double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),
@@ -144,6 +153,30 @@ void hwt_plan_destroy(hwt_plan *plan);
  * sequences back to back. */
 int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
                    int64_t *alpha_offsets, double *alphas);
+
+/* Project autotuner (no counterpart in the reference, whose runners take a
+ * block size and sort order from the command line, cuda-option/Kernel.cuh:303
+ * and cuda-multi/Kernel.cuh:374).  Prices the device batch repeatedly with
+ * candidate settings of form 2 -- processing order of the wide trees (work,
+ * height or width), narrow trees four or one per wavefront, the two pricing
+ * kernels side by side or in sequence, and resident workgroups per CU of
+ * each -- by greedy coordinate descent from *cfg, timing each whole pipeline
+ * with HIP events (median of three), and writes the fastest back into *cfg.
+ * The choice is cached per batch signature (precision, option count and the
+ * tree-class counts and total work, each to a power of two) for the life of
+ * the process, and appended to the file named by the environment variable
+ * HWT_TUNE_CACHE when it is set (read at the first call).  dev_out receives
+ * the prices of the last run. */
+typedef struct {
+    double  ms;          /* pipeline time of the chosen configuration */
+    double  default_ms;  /* ... of the configuration passed in */
+    double  tune_ms;     /* wall time spent tuning (0 on a cache hit) */
+    int32_t candidates;  /* configurations timed */
+    int32_t cached;      /* 1 if the choice came from the cache */
+} hwt_tune_report;
+
+int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64 *dev_yield, double *dev_out,
+                 hwt_config *cfg, hwt_tune_report *report);
 
 /* Largest tree width (2*jmax+1) each form supports. */
 int32_t hwt_max_width(int32_t form);

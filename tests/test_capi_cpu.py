@@ -50,6 +50,10 @@ def test_struct_layouts_match_header(pkg):
     assert C.sizeof(pkg.capi.OptionsC) == 8 + 8 * 8
     assert C.sizeof(pkg.capi.YieldC) == 24
     assert C.sizeof(pkg.capi.ConfigC) == 8 * 4
+    assert C.sizeof(pkg.capi.TuneReportC) == 3 * 8 + 2 * 4
+    text = (ROOT / "include" / "hwtree.h").read_text()
+    for name, _ in pkg.capi.ConfigC._fields_:
+        assert re.search(rf"int32_t\s+{name};", text), name
     assert C.sizeof(pkg.capi.StatsC) == 8 * 8 + 4 * 4
     assert C.sizeof(pkg.surrogate.SurrConfigC) == 10 * 4 + 8 + 6 * 4
     assert C.sizeof(pkg.surrogate.SurrStatsC) == 3 * 8 + 2 * 4 + 2 * 8 + 4 * 4
@@ -76,5 +80,6 @@ def test_null_arguments_rejected_without_gpu(pkg):
     assert lib.hwt_price_f64(None, None, None, None, None) == pkg.capi.HWT_EINVAL
     assert lib.hwt_plan_create(None, 10, 10, None) == pkg.capi.HWT_EINVAL
     assert lib.hwt_plan_sync(None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_autotune(None, None, None, None, None) == pkg.capi.HWT_EINVAL
     lib.hwt_plan_destroy(None)  # no-op
     assert b"hwt_" in lib.hwt_last_error()

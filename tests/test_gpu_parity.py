@@ -241,3 +241,33 @@ def test_yield_curve_shapes(pkg, oracle, curve):
         y = pkg.formats.Yield(np.array([0.025]), np.array([365], np.int32))
     opts = pkg.Options.concat([pkg.datasets.book(), pkg.datasets.skewed(seed=9, n=200)])
     assert_parity(pkg.capi.price(opts, y), oracle.price(opts, y), curve)
+
+
+@pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"wide_per_cu": 2, "narrow_per_cu": 4},
+                                     {"sort": "H"}, {"sort": "W"}])
+def test_form2_settings_price_the_same(pkg, oracle, setting):
+    """Every setting the autotuner can choose prices within the parity bar."""
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.Options.concat([pkg.datasets.book(), pkg.datasets.skewed(seed=17, n=1500)])
+    kw = dict(setting)
+    cfg = pkg.capi.config(sort=kw.pop("sort", "L"), flags=kw.pop("flags", 0))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    got = pkg.capi.price(opts, y, cfg)
+    assert_parity(got, oracle.price(opts, y), str(setting))
+
+
+def test_autotune_chooses_and_caches(pkg, oracle):
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=13, n=4000)
+    dev = pkg.capi.to_device(opts, y)
+    cfg, rep = pkg.capi.autotune(dev)
+    assert rep["candidates"] >= 2 and rep["cached"] == 0
+    assert 0 < rep["ms"] <= rep["default_ms"]
+    tuned = pkg.capi.price(opts, y, cfg)
+    sub = np.arange(0, opts.N, 40)
+    assert_parity(tuned[sub], oracle.price(opts.take(sub), y), "tuned configuration")
+    # the same batch shape again: the cached choice, no search
+    cfg2, rep2 = pkg.capi.autotune(dev)
+    assert rep2["cached"] == 1 and rep2["candidates"] == 1
+    assert cfg2.as_dict() == cfg.as_dict()
