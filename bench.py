@@ -87,9 +87,16 @@ def surrogate_leg(pkg, y, args):
     ev = S.evaluate(gated, y, n=1_000_000, seed=5, reps=max(1, min(args.steps, 10)))
     gated.close()
     s.close()
+    flop = ev["options"] * 4 * 2 * (8 * 256 + 3 * 256 * 256 + 256)
+    achieved = flop / (ev["inference_ms"] * 1e-3) / 1e12
     return {"workload": "1,000,000 options: 95% inside the training box, 5% outside (a in [0.2,0.4] "
                         "or expiry 30-40y); ensemble BF16 MLP inference + FP64 tree fallback",
-            "training": fit, "calibration": cal, **ev}
+            "training": fit, "calibration": cal, **ev,
+            "roofline": {"bound": "mfma", "kernel": "k_infer", "achieved": achieved, "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+                         "flop_per_launch": flop,
+                         "flop_model": "4 members x 2 x (8*256 + 3*256*256 + 256) FLOP per option, every row "
+                                       "of every tile; BF16 MFMA dense peak"}}
 
 
 def main():
@@ -103,7 +110,15 @@ def main():
                     help="training epochs of the surrogate ensemble over its 2M tree-priced samples")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: one 64 MiB device-to-device copy (the byte-counter calibration "
-                         "dispatch), then warmup + steps executions of the pipeline without the graph")
+                         "dispatch), then warmup + steps executions of the pipeline without the graph, "
+                         "with the settings given by --sort/--serial/--no-rows (no tuning runs, so every "
+                         "traced pricing dispatch is of the profiled configuration)")
+    ap.add_argument("--sort", default="L", help="--profile: processing order of the wide trees")
+    ap.add_argument("--serial", action="store_true", help="--profile: narrow-tree kernel after the wide one")
+    ap.add_argument("--no-rows", action="store_true", help="--profile: narrow trees one per wavefront")
+    ap.add_argument("--profile-surrogate", action="store_true",
+                    help="profiler run of the surrogate: a short training (2 epochs on 262,144 tree-priced "
+                         "samples), then warmup + steps inference-with-fallback passes over the 1M-option mix")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): options are independent, "
@@ -115,6 +130,19 @@ def main():
     pkg = ge.load_package()
     capi, dm = pkg.capi, pkg.devmem
     y = pkg.datasets.synthetic_yield()
+    if args.profile_surrogate:
+        S = pkg.surrogate
+        s, _ = S.fit(y, n_train=262144, epochs=2, seed=11)
+        mix = pkg.datasets.box_mix(5, 1_000_000)
+        dmix = capi.to_device(mix, y)
+        fb = capi.Plan(mix.N, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+        for i in range(args.warmup + args.steps):
+            st = s.price(fb, dmix)
+            print(f"profile surrogate pass {i}: {st['ms']:.3f} ms (inference {st['surrogate_ms']:.3f} ms, "
+                  f"fallback {st['fallback']})", flush=True)
+        fb.close()
+        s.close()
+        return
     opts = pkg.datasets.skewed_60k()
     dev = capi.to_device(opts, y)
 
@@ -123,7 +151,9 @@ def main():
         src, dst = dm.DeviceArray(n, np.uint8, (n,)), dm.DeviceArray(n, np.uint8, (n,))
         dm._ck(dm.hip().hipMemcpy(dst.ptr, src.ptr, n, dm.D2D))
         dm.synchronize()
-        plan = capi.Plan(opts.N, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+        flags = capi.FLAG_NO_GRAPH | (capi.FLAG_SERIAL if args.serial else 0) | \
+            (capi.FLAG_NO_ROWS if args.no_rows else 0)
+        plan = capi.Plan(opts.N, y.N, capi.config(sort=args.sort, flags=flags))
         wide = []
         for i in range(args.warmup + args.steps):
             plan.execute(dev)
