@@ -111,10 +111,10 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: one 64 MiB device-to-device copy (the byte-counter calibration "
                          "dispatch), then warmup + steps executions of the pipeline without the graph, "
-                         "with the settings given by --sort/--serial/--no-rows (no tuning runs, so every "
+                         "with the settings given by --sort/--concurrent/--no-rows (no tuning runs, so every "
                          "traced pricing dispatch is of the profiled configuration)")
     ap.add_argument("--sort", default="L", help="--profile: processing order of the wide trees")
-    ap.add_argument("--serial", action="store_true", help="--profile: narrow-tree kernel after the wide one")
+    ap.add_argument("--concurrent", action="store_true", help="--profile: narrow-tree kernel beside the wide one")
     ap.add_argument("--no-rows", action="store_true", help="--profile: narrow trees one per wavefront")
     ap.add_argument("--profile-surrogate", action="store_true",
                     help="profiler run of the surrogate: a short training (2 epochs on 262,144 tree-priced "
@@ -151,7 +151,7 @@ def main():
         src, dst = dm.DeviceArray(n, np.uint8, (n,)), dm.DeviceArray(n, np.uint8, (n,))
         dm._ck(dm.hip().hipMemcpy(dst.ptr, src.ptr, n, dm.D2D))
         dm.synchronize()
-        flags = capi.FLAG_NO_GRAPH | (capi.FLAG_SERIAL if args.serial else 0) | \
+        flags = capi.FLAG_NO_GRAPH | (capi.FLAG_CONCURRENT if args.concurrent else 0) | \
             (capi.FLAG_NO_ROWS if args.no_rows else 0)
         plan = capi.Plan(opts.N, y.N, capi.config(sort=args.sort, flags=flags))
         wide = []

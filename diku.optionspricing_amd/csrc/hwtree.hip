@@ -680,9 +680,9 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
             HWT_HIP(hipGetLastError());
             return HWT_OK;
         }
-        // the narrow-tree kernel beside the wide-tree one on the side stream,
-        // or after it on `s` (HWT_FLAG_SERIAL)
-        const bool serial = (p->cfg.flags & HWT_FLAG_SERIAL) != 0;
+        // the narrow-tree kernel after the wide-tree one on `s`, or beside it
+        // on the side stream (HWT_FLAG_CONCURRENT)
+        const bool serial = (p->cfg.flags & HWT_FLAG_CONCURRENT) == 0;
         hipStream_t ns = serial ? s : p->side;
         if (!serial) {
             HWT_HIP(hipEventRecord(p->fork, s));
@@ -1160,7 +1160,7 @@ void tune_store_file(const std::string &key, const hwt_config &c)
     const char *path = getenv("HWT_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL)) << ' ' << c.wide_per_cu << ' '
+    f << key << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) << ' ' << c.wide_per_cu << ' '
       << c.narrow_per_cu << '\n';
 }
 
@@ -1219,8 +1219,8 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         if (it != g_tune_cache.end()) {
             hwt_config c = base;
             c.sort = it->second.sort;
-            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL)) |
-                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_SERIAL));
+            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) |
+                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT));
             c.wide_per_cu = it->second.wide_per_cu;
             c.narrow_per_cu = it->second.narrow_per_cu;
             *cfg = c;
@@ -1255,7 +1255,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         c.sort = so;
         if ((rc = consider(c))) return rc;
     }
-    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_SERIAL}) {
+    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_CONCURRENT}) {
         hwt_config c = best;
         c.flags ^= bit;
         if ((rc = consider(c))) return rc;

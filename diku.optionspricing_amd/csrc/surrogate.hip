@@ -125,8 +125,9 @@ SOpts sopts(const hwt_options_f64 &o)
 // B~ = dt (1 - q^m) / (1 - q), q = e^{-a dt}, m = n - e (discrete bond
 // sensitivity); c0 = ln(100 P(S)/P(T)) minus the continuous-time convexity
 // terms B sigma^2 (1-e^{-aT})^2 / (2a^2) + B^2 sigma^2 (1 - e^{-2aT}) / (4a).
-template <class YieldFn>
-__device__ bool surr_inputs(const SOpts &o, int i, float x[NIN], double &base, const YieldFn &yield)
+// Whether option i lies inside the training box with an exercise step inside
+// its tree; c receives its tree constants.
+__device__ bool surr_in_box(const SOpts &o, int i, OptConst &c)
 {
     const double X = o.strike[i], Tb = o.maturity[i], L = o.length[i], a = o.a[i], sg = o.sigma[i];
     const int tu = o.tu[i], st = o.steps[i], ty = o.type[i];
@@ -134,10 +135,17 @@ __device__ bool surr_inputs(const SOpts &o, int i, float x[NIN], double &base, c
                      Tb - L >= 1.0 && Tb - L <= 10.0 && a >= 0.01 && a <= 0.2 && sg >= 0.005 && sg <= 0.02 &&
                      (ty == 0 || ty == 1);
     if (!box) return false;
+    return option_constants(X, Tb, L, (uint16_t)tu, (uint16_t)st, a, sg, (int8_t)ty, 1 << 30, c) == ST_OK &&
+           c.e >= 1 && c.n > c.e;
+}
+
+template <class YieldFn>
+__device__ bool surr_inputs(const SOpts &o, int i, float x[NIN], double &base, const YieldFn &yield)
+{
+    const double X = o.strike[i], a = o.a[i], sg = o.sigma[i];
+    const int tu = o.tu[i], st = o.steps[i], ty = o.type[i];
     OptConst c;
-    if (option_constants(X, Tb, L, (uint16_t)tu, (uint16_t)st, a, sg, (int8_t)ty, 1 << 30, c) != ST_OK || c.e < 1 ||
-        c.n <= c.e)
-        return false;
+    if (!surr_in_box(o, i, c)) return false;
     const double T = c.e * c.dt, S = c.n * c.dt;
     const double PT = exp(-yield(T, tu) * T);
     const double PS = exp(-yield(S, tu) * S);
@@ -539,6 +547,8 @@ struct InferArgs {
     int8_t *in_box;    // (predict) or null
     int32_t *fb_idx;   // fallback list or null
     int32_t *fb_count; // [0] total, [1] out of box
+    const int32_t *rows;      // option of each row (null: row i is option i)
+    const int32_t *row_count; // number of listed rows (with `rows`)
 };
 
 __device__ inline uint32_t pack_bf16x2(float a, float b) { return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16); }
@@ -558,6 +568,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __shared__ int32_t yd[INF_YMAX];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int r0 = blockIdx.x * INF_ROWS;
+    const int nrows = g.rows ? *g.row_count : g.o.n;
+    if (r0 >= nrows) return;  // the whole tile is past the listed rows
 
     // curves that fit are read from LDS, by binary search when their days
     // never decrease; both lookups give the reference's value
@@ -572,10 +584,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     const bool sorted = staged && !__syncthreads_or(unsorted);
     if (t < INF_ROWS) {
-        const int i = r0 + t;
+        const int i = r0 + t < nrows ? (g.rows ? g.rows[r0 + t] : r0 + t) : -1;
         float x[NIN] = {};
         double base = 0.0;
-        const bool ok = i < g.o.n && surr_inputs(g.o, i, x, base, [&](double tt, int tu) {
+        const bool ok = i >= 0 && surr_inputs(g.o, i, x, base, [&](double tt, int tu) {
             return sorted ? yield_at_sorted(tt, tu, yr, yd, g.y.n) : yield_at(tt, tu, g.y.rate, g.y.day, g.y.n);
         });
 #pragma unroll
@@ -689,8 +701,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     }
     if (t < INF_ROWS) {
-        const int i = r0 + t;
-        if (i < g.o.n) {
+        const int i = r0 + t < nrows ? (g.rows ? g.rows[r0 + t] : r0 + t) : -1;
+        if (i >= 0) {
             const double mean = bases[t] + RES * (double)(ysum[t] / (float)g.members);
             const double spread = RES * (double)(ymax[t] - ymin[t]);
             const bool ok = okf[t] != 0;
@@ -714,6 +726,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();
         if (t < fb_n) g.fb_idx[fb_base + t] = fb_local[t];
     }
+}
+
+// Splits a batch before inference: options outside the training box go to
+// the fallback list (fb_idx, fb_count[0] and [1]), the others to the row list
+// of the inference kernel (rows, row_count).  One global atomic per block and
+// list; order inside the lists is arbitrary.
+__global__ __launch_bounds__(256) void k_box(SOpts o, int32_t *__restrict__ fb_idx, int32_t *__restrict__ fb_count,
+                                             int32_t *__restrict__ rows, int32_t *__restrict__ row_count)
+{
+    __shared__ int32_t nin, nout, bin, bout;
+    __shared__ int32_t lin[256], lout[256];
+    const int t = threadIdx.x, i = blockIdx.x * 256 + t;
+    if (t == 0) nin = nout = 0;
+    __syncthreads();
+    if (i < o.n) {
+        OptConst c;
+        if (surr_in_box(o, i, c)) lin[atomicAdd(&nin, 1)] = i;
+        else lout[atomicAdd(&nout, 1)] = i;
+    }
+    __syncthreads();
+    if (t == 0) {
+        bin = nin ? atomicAdd(row_count, nin) : 0;
+        bout = nout ? atomicAdd(&fb_count[0], nout) : 0;
+        if (nout) atomicAdd(&fb_count[1], nout);
+    }
+    __syncthreads();
+    if (t < nin) rows[bin + t] = lin[t];
+    if (t < nout) fb_idx[bout + t] = lout[t];
 }
 
 // Fallback list -> compact option arrays, and prices back to input order.
@@ -797,7 +837,9 @@ struct hwt_surrogate {
     int32_t *perm = nullptr;  // [members][train_cap]
     int32_t *rows = nullptr;  // test hook rows [B]
     int32_t *bad = nullptr;
-    int32_t *fb_idx = nullptr, *fb_count = nullptr;
+    int32_t *fb_idx = nullptr;   // fallback list: out-of-box options, then uncertain in-box ones
+    int32_t *inf_rows = nullptr; // in-box options, the rows of the inference kernel
+    int32_t *fb_count = nullptr; // [0] fallback total, [1] of which out of box, [4] inference rows
     double *fb_tmp = nullptr;
     Compact comp{};
     int32_t *h_count = nullptr;  // pinned [2]
@@ -1005,9 +1047,8 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
     if (hipGetDevice(&s->device) != hipSuccess) return bail(HWT_EHIP, "hipGetDevice failed");
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(HWT_EHIP, "hipStreamCreate failed");
-    if (hipEventCreate(&s->e0) != hipSuccess || hipEventCreate(&s->e1) != hipSuccess ||
-        hipEventCreate(&s->e2) != hipSuccess)
-        return bail(HWT_EHIP, "hipEventCreate failed");
+    for (hipEvent_t *e : {&s->e0, &s->e1, &s->e2})
+        if (hipEventCreate(e) != hipSuccess) return bail(HWT_EHIP, "hipEventCreate failed");
     const size_t M = s->members, B = s->B, T = std::max(train_capacity, 1), I = std::max(infer_capacity, 1);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
@@ -1023,7 +1064,8 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
     const size_t oy = carve(M * B * 4), ogw = carve(M * 3 * DW_SPLIT * LAYER_W * 4), ogs = carve(M * PSTRIDE * 4);
     const size_t ohp = carve(M * (B / HEAD_ROWS) * (HID + 1) * 4), oloss = carve(M * 8), og = carve(M * PSTRIDE * 4);
     const size_t oX = carve(T * NIN * 4), otg = carve(T * 4), operm = carve(M * T * 4), orows = carve(B * 4);
-    const size_t obad = carve(4), ofbi = carve(I * 4), ofbc = carve(8), ofbt = carve(I * 8);
+    const size_t obad = carve(4), ofbi = carve(I * 4), ofbc = carve(8 * 4), ofbt = carve(I * 8);
+    const size_t oinr = carve(I * 4);
     const size_t ocs = carve(I * 8), ocm = carve(I * 8), ocl = carve(I * 8), oca = carve(I * 8), ocg = carve(I * 8);
     const size_t ocu = carve(I * 2), ocst = carve(I * 2), octy = carve(I);
     s->bytes = off;
@@ -1057,11 +1099,12 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
     s->rows = (int32_t *)(b + orows);
     s->bad = (int32_t *)(b + obad);
     s->fb_idx = (int32_t *)(b + ofbi);
+    s->inf_rows = (int32_t *)(b + oinr);
     s->fb_count = (int32_t *)(b + ofbc);
     s->fb_tmp = (double *)(b + ofbt);
     s->comp = Compact{(double *)(b + ocs), (double *)(b + ocm), (double *)(b + ocl), (double *)(b + oca),
                       (double *)(b + ocg), (uint16_t *)(b + ocu), (uint16_t *)(b + ocst), (int8_t *)(b + octy)};
-    if (hipHostMalloc((void **)&s->h_count, 2 * sizeof(int32_t), 0) != hipSuccess ||
+    if (hipHostMalloc((void **)&s->h_count, 8 * sizeof(int32_t), 0) != hipSuccess ||
         hipHostMalloc((void **)&s->h_loss, M * sizeof(double), 0) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     // zero everything once (gradient partials of unused rows stay defined)
@@ -1255,8 +1298,14 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
     DeviceGuardS dg(s->device);
     hipStream_t st = s->stream;
     SURR_HIP(hipEventRecord(s->e0, st));
-    k_zero_i32<<<1, 32, 0, st>>>(s->fb_count, 2);
+    k_zero_i32<<<1, 32, 0, st>>>(s->fb_count, 8);
     if (o->n > 0) {
+        const int blocks = (o->n + 255) / 256;
+        // split: out-of-box options to the fallback list, in-box ones to the
+        // inference rows (running the tree on the out-of-box part beside the
+        // inference was measured slower: both compete for the same CUs)
+        k_box<<<blocks, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->inf_rows, s->fb_count + 4);
+        // the ensemble on the in-box rows; options it is unsure of join the list
         InferArgs g{};
         g.o = sopts(*o);
         g.y = SYield{y->n, y->rate, y->day};
@@ -1266,22 +1315,24 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
         g.out = out;
         g.fb_idx = s->fb_idx;
         g.fb_count = s->fb_count;
+        g.rows = s->inf_rows;
+        g.row_count = s->fb_count + 4;
         k_infer<<<(o->n + INF_ROWS - 1) / INF_ROWS, 256, 0, st>>>(g);
-    }
-    SURR_HIP(hipGetLastError());
-    SURR_HIP(hipEventRecord(s->e1, st));
-    if (o->n > 0) {
-        const int blocks = (o->n + 255) / 256;
+        SURR_HIP(hipGetLastError());
+        SURR_HIP(hipEventRecord(s->e1, st));
+        // the whole list on the FP64 tree in one pass
         k_fb_gather<<<blocks, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->comp);
-        hwt_options_f64 co{o->n,          s->comp.strike, s->comp.maturity, s->comp.length, s->comp.tu,
-                           s->comp.steps, s->comp.a,      s->comp.sigma,    s->comp.type};
+        const hwt_options_f64 co{o->n,          s->comp.strike, s->comp.maturity, s->comp.length, s->comp.tu,
+                                 s->comp.steps, s->comp.a,      s->comp.sigma,    s->comp.type};
         const int rc = hwt_plan_execute_counted(tree, &co, s->fb_count, y, s->fb_tmp, st);
         if (rc) return rc;
         k_fb_scatter<<<blocks, 256, 0, st>>>(s->fb_idx, s->fb_count, s->fb_tmp, out);
         SURR_HIP(hipGetLastError());
+    } else {
+        SURR_HIP(hipEventRecord(s->e1, st));
     }
     SURR_HIP(hipEventRecord(s->e2, st));
-    SURR_HIP(hipMemcpyAsync(s->h_count, s->fb_count, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SURR_HIP(hipMemcpyAsync(s->h_count, s->fb_count, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SURR_HIP(hipStreamSynchronize(st));
     const int trc = hwt_plan_sync(tree, nullptr);
     if (stats) {

@@ -266,7 +266,16 @@ def with_threshold(s: Surrogate, threshold: float, infer_capacity: int, yield_ca
     return t
 
 
-def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, log=None) -> dict:
+def fallback_plan(n: int, yield_capacity: int, wide_per_cu: int = 0, narrow_per_cu: int = 0) -> capi.Plan:
+    """The tree plan hwt_surr_price re-prices with (no graph; optionally fewer
+    resident workgroups per CU, leaving room for the inference beside it)."""
+    cfg = capi.config(flags=capi.FLAG_NO_GRAPH)
+    cfg.wide_per_cu, cfg.narrow_per_cu = wide_per_cu, narrow_per_cu
+    return capi.Plan(n, yield_capacity, cfg)
+
+
+def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, log=None,
+             fallback_per_cu: tuple[int, int] = (0, 0)) -> dict:
     """Surrogate + tree fallback against the plain FP64 tree on the mixed
     batch (95% inside the box, 5% outside): options/s of each (HIP events),
     the fallback fraction, and the error of every price against the tree's."""
@@ -274,10 +283,12 @@ def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, 
     opts = datasets.box_mix(seed, n)
     dev = capi.to_device(opts, y)
 
-    # plain tree: the plan's HIP graph, timed with events on a caller stream
+    # plain tree at its best: the autotuned configuration, the plan's HIP
+    # graph, timed with events on a caller stream
+    tuned, _ = capi.autotune(dev)
     stream = C.c_void_p()
     devmem._ck(devmem.hip().hipStreamCreate(C.byref(stream)))
-    plan = capi.Plan(n, y.N)
+    plan = capi.Plan(n, y.N, tuned)
     plan.execute(dev, stream=stream.value)
     plan.sync()
     tree = dev["out"].to_host()
@@ -291,7 +302,7 @@ def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, 
     tree_ms = e0.elapsed_ms(e1) / reps
     plan.close()
 
-    fb_plan = capi.Plan(n, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+    fb_plan = fallback_plan(n, y.N, *fallback_per_cu)
     out = devmem.DeviceArray.empty((n,), np.float64)
     s.price(fb_plan, dev, out)  # warm-up
     runs = [s.price(fb_plan, dev, out) for _ in range(reps)]
@@ -317,6 +328,7 @@ def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, 
         "tree_ms": tree_ms, "tree_options_per_s": n / (tree_ms * 1e-3),
         "surrogate_ms": ms, "surrogate_options_per_s": n / (ms * 1e-3),
         "speedup": tree_ms / ms,
+        "tree_config": tuned.as_dict(),
         "inference_ms": float(np.mean([r["surrogate_ms"] for r in runs])),
         "fallback_tree_ms": float(np.mean([r["tree_ms"] for r in runs])),
         "fallback": int(runs[-1]["fallback"]), "out_of_box": int(runs[-1]["out_of_box"]),
@@ -326,6 +338,7 @@ def evaluate(s: Surrogate, y, n: int = 1_000_000, seed: int = 5, reps: int = 5, 
         "in_box_network_max_abs_err": float(err_net.max()) if err_net.size else 0.0,
         "in_box_network_mean_abs_err": float(err_net.mean()) if err_net.size else 0.0,
         "spread_threshold": float(s.cfg.spread_threshold),
+        "fallback_per_cu": list(fallback_per_cu),
         "options_above_1e-3": int((err > 1e-3).sum()),
         "in_box_closed_form_max_abs_err": float(np.abs(base[inbox] - tree[inbox]).max()) if inbox.any() else 0.0,
         "in_box_closed_form_mean_abs_err": float(np.abs(base[inbox] - tree[inbox]).mean()) if inbox.any() else 0.0,

@@ -101,9 +101,9 @@ typedef struct {
 /* Form 2: price trees up to 64 nodes wide one per wavefront instead of four
  * per wavefront (16-lane rows). */
 #define HWT_FLAG_NO_ROWS  2
-/* Form 2: run the narrow-tree kernel after the wide-tree kernel on one stream
- * instead of beside it on a second stream. */
-#define HWT_FLAG_SERIAL   4
+/* Form 2: run the narrow-tree kernel beside the wide-tree kernel on a second
+ * stream instead of after it on the same stream. */
+#define HWT_FLAG_CONCURRENT 4
 
 typedef struct {
     double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),
@@ -159,7 +159,7 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
  * and cuda-multi/Kernel.cuh:374).  Prices the device batch repeatedly with
  * candidate settings of form 2 -- processing order of the wide trees (work,
  * height or width), narrow trees four or one per wavefront, the two pricing
- * kernels side by side or in sequence, and resident workgroups per CU of
+ * kernels in sequence or side by side, and resident workgroups per CU of
  * each -- by greedy coordinate descent from *cfg, timing each whole pipeline
  * with HIP events (median of three), and writes the fastest back into *cfg.
  * The choice is cached per batch signature (precision, option count and the

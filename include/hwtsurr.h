@@ -75,7 +75,7 @@ typedef struct {
     int64_t steps;            /* optimiser steps taken so far */
     int32_t fallback;         /* options re-priced by the tree (hwt_surr_price) */
     int32_t out_of_box;       /* ... of which outside the training box */
-    double  surrogate_ms;     /* features + ensemble inference + gate (hwt_surr_price) */
+    double  surrogate_ms;     /* box split + features + ensemble inference + gate (hwt_surr_price) */
     double  tree_ms;          /* fallback tree pricing (hwt_surr_price) */
     int32_t reserved[4];
 } hwt_surr_stats;
@@ -104,11 +104,13 @@ int hwt_surr_train(hwt_surrogate *s, int32_t epochs, int32_t total_epochs, hwt_s
 int hwt_surr_predict(hwt_surrogate *s, const hwt_options_f64 *dev_opts, const hwt_yield_f64 *dev_yield,
                      double *dev_mean, double *dev_spread, int8_t *dev_in_box);
 
-/* Surrogate with fallback: prices in input order into dev_out.  Options out
- * of the box or with spread above the threshold are compacted and priced by
- * the FP64 tree in the same stream (device-side count, no host round trip).
- * `tree` is a plan of capacity >= infer_capacity created with
- * HWT_FLAG_NO_GRAPH. */
+/* Surrogate with fallback: prices in input order into dev_out.  A first
+ * kernel splits the batch: options outside the box go to the fallback list,
+ * the ensemble runs on the in-box options only, and those whose spread
+ * exceeds the threshold join the list; the list is then compacted and priced
+ * by the FP64 tree in one pass, in the same stream.  Counts stay on the
+ * device (no host round trip).  `tree` is a plan of capacity >=
+ * infer_capacity created with HWT_FLAG_NO_GRAPH. */
 int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *dev_opts,
                    const hwt_yield_f64 *dev_yield, double *dev_out, hwt_surr_stats *stats);
 
