@@ -529,10 +529,11 @@ __global__ void k_refresh_bf16(Net net, int members)
 // of one row in each lane's accumulator quad and the epilogue writes them as
 // one 8-byte store.  Wave w owns neurons [64w, 64w + 64) for all 128 rows,
 // so each weight is read once per tile; a layer reads the whole tile, a
-// barrier, then the epilogue overwrites it in place.  Layer 1 (FP32, the
-// training kernel's order) reads its wave-uniform weights through the scalar
-// cache; the output layer reduces per row across lanes and waves in a fixed
-// order.  The yield curve is staged in LDS for the inputs.
+// barrier, then the epilogue overwrites it in place.  Layer 1 runs the same
+// way on the FP32 matrix instructions (K = 8 in four 32x32x2 steps); every
+// accumulator starts at its bias.  The output layer reduces per row across
+// lanes and waves in a fixed order.  The yield curve is staged in LDS for the
+// inputs.
 constexpr int INF_YMAX = 256;  // keeps two 128-row tiles per CU in LDS
 
 struct InferArgs {
@@ -552,6 +553,47 @@ struct InferArgs {
 };
 
 __device__ inline uint32_t pack_bf16x2(float a, float b) { return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16); }
+
+// Accumulators of a wave's 2 x 4 blocks D[neuron][row] seeded with the
+// layer's bias: lane l's value v belongs to neuron 32 mb + 8 (v/4) + 4 (l/32)
+// + v%4 of the wave's 64, whatever the row block.
+__device__ inline void bias_init(f32x16 (&acc)[2][4], const float *bias, int n0, int lane)
+{
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        f32x16 b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 bv = *reinterpret_cast<const float4 *>(bias + n0 + 32 * mb + 8 * q + 4 * (lane >> 5));
+            b[4 * q] = bv.x;
+            b[4 * q + 1] = bv.y;
+            b[4 * q + 2] = bv.z;
+            b[4 * q + 3] = bv.w;
+        }
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = b;
+    }
+}
+
+// SiLU of the pre-activations, rounded to BF16, into the activation tile:
+// the lane's four consecutive neurons of one row as one 8-byte store.
+__device__ inline void silu_to_act(const f32x16 (&acc)[2][4], uint16_t *act, int n0, int lane)
+{
+    const int mrow = lane & 31;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = n0 + 32 * mb + 8 * q + 4 * (lane >> 5);
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const f32x16 &a = acc[mb][nb];
+                const uint2 pk = make_uint2(pack_bf16x2(silu(a[4 * q]), silu(a[4 * q + 1])),
+                                            pack_bf16x2(silu(a[4 * q + 2]), silu(a[4 * q + 3])));
+                *reinterpret_cast<uint2 *>(act + (32 * nb + mrow) * ACT_LD + col) = pk;
+            }
+        }
+}
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_infer(InferArgs g)
 {
@@ -605,23 +647,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int n0 = wave * 64;  // this wave's output neurons in the hidden layers
     for (int m = 0; m < g.members; ++m) {
         const float *P = g.net.P + (size_t)m * PSTRIDE;
-        // layer 1: thread t -> row t % 128, neurons [128 (t / 128), +128)
+        // layer 1 (FP32 MFMA 32x32x2, K = 8 in four steps): D[neuron][row] =
+        // b1 + W1 x^T, with the lane's A value W1[n][k] and B value x[row][k]
         {
-            const int r = t & (INF_ROWS - 1), c0 = (t >> 7) * 128;
-            float x[NIN];
+            f32x16 acc[2][4];
+            bias_init(acc, P + HWT_SURR_B1, n0, lane);
 #pragma unroll
-            for (int k = 0; k < NIN; ++k) x[k] = xs[r * NIN + k];
-            for (int c = c0; c < c0 + 128; c += 4) {
-                float v[4];
+            for (int k = 0; k < NIN; k += 2) {
+                const int kk = k + (lane >> 5);
+                const float a0 = P[HWT_SURR_W1 + (n0 + mrow) * NIN + kk];
+                const float a1 = P[HWT_SURR_W1 + (n0 + 32 + mrow) * NIN + kk];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float acc = P[HWT_SURR_B1 + c + j];
-#pragma unroll
-                    for (int k = 0; k < NIN; ++k) acc = fmaf(P[HWT_SURR_W1 + (c + j) * NIN + k], x[k], acc);
-                    v[j] = silu(acc);
+                for (int nb = 0; nb < 4; ++nb) {
+                    const float bx = xs[(32 * nb + mrow) * NIN + kk];
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bx, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bx, acc[1][nb], 0, 0, 0);
                 }
-                *reinterpret_cast<uint2 *>(act + r * ACT_LD + c) = make_uint2(pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]));
             }
+            silu_to_act(acc, act, n0, lane);
         }
         __syncthreads();
         for (int l = 2; l <= 4; ++l) {
@@ -630,10 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const uint16_t *wa1 = wa0 + (size_t)32 * HID;
             const uint16_t *ab = act + mrow * ACT_LD + khalf;
             f32x16 acc[2][4];
-#pragma unroll
-            for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x16{};
+            bias_init(acc, P + boff(l), n0, lane);
 #pragma unroll 2
             for (int k = 0; k < HID; k += 16) {
                 const bf16x8 a0 = load_frag(wa0 + k), a1 = load_frag(wa1 + k);
@@ -646,22 +686,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 }
             }
             __syncthreads();  // every wave has read this layer's input tile
-            const float *bias = P + boff(l);
             if (l < 4) {
-#pragma unroll
-                for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int col = n0 + 32 * mb + 8 * q + 4 * (lane >> 5);
-                        const float4 bv = *reinterpret_cast<const float4 *>(bias + col);
-#pragma unroll
-                        for (int nb = 0; nb < 4; ++nb) {
-                            const f32x16 &a = acc[mb][nb];
-                            const uint2 pk = make_uint2(pack_bf16x2(silu(a[4 * q] + bv.x), silu(a[4 * q + 1] + bv.y)),
-                                                        pack_bf16x2(silu(a[4 * q + 2] + bv.z), silu(a[4 * q + 3] + bv.w)));
-                            *reinterpret_cast<uint2 *>(act + (32 * nb + mrow) * ACT_LD + col) = pk;
-                        }
-                    }
+                silu_to_act(acc, act, n0, lane);
                 __syncthreads();
             } else {
                 // output layer in FP32 from the accumulators: per row, the
@@ -673,15 +699,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int col = n0 + 32 * mb + 8 * q + 4 * (lane >> 5);
-                        const float4 bv = *reinterpret_cast<const float4 *>(bias + col);
                         const float4 w5 = *reinterpret_cast<const float4 *>(P + HWT_SURR_W5 + col);
 #pragma unroll
                         for (int nb = 0; nb < 4; ++nb) {
                             const f32x16 &a = acc[mb][nb];
-                            p[nb] = fmaf(w5.x, silu(a[4 * q] + bv.x), p[nb]);
-                            p[nb] = fmaf(w5.y, silu(a[4 * q + 1] + bv.y), p[nb]);
-                            p[nb] = fmaf(w5.z, silu(a[4 * q + 2] + bv.z), p[nb]);
-                            p[nb] = fmaf(w5.w, silu(a[4 * q + 3] + bv.w), p[nb]);
+                            p[nb] = fmaf(w5.x, silu(a[4 * q]), p[nb]);
+                            p[nb] = fmaf(w5.y, silu(a[4 * q + 1]), p[nb]);
+                            p[nb] = fmaf(w5.z, silu(a[4 * q + 2]), p[nb]);
+                            p[nb] = fmaf(w5.w, silu(a[4 * q + 3]), p[nb]);
                         }
                     }
 #pragma unroll
