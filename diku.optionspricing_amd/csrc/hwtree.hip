@@ -1152,7 +1152,7 @@ void tune_load_file()
     std::string key;
     hwt_config c;
     hwt_config_init(&c);
-    while (f >> key >> c.sort >> c.flags >> c.wide_per_cu >> c.narrow_per_cu) g_tune_cache[key] = c;
+    while (f >> key >> c.form >> c.sort >> c.flags >> c.wide_per_cu >> c.narrow_per_cu) g_tune_cache[key] = c;
 }
 
 void tune_store_file(const std::string &key, const hwt_config &c)
@@ -1160,8 +1160,8 @@ void tune_store_file(const std::string &key, const hwt_config &c)
     const char *path = getenv("HWT_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) << ' ' << c.wide_per_cu << ' '
-      << c.narrow_per_cu << '\n';
+    f << key << ' ' << c.form << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) << ' '
+      << c.wide_per_cu << ' ' << c.narrow_per_cu << '\n';
 }
 
 // Median pipeline time of three timed executions after one warm-up.
@@ -1198,7 +1198,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         return fail(HWT_EINVAL, "hwt_autotune: null or negative-size argument");
     const auto w0 = std::chrono::steady_clock::now();
     hwt_tune_report rep{};
-    if (cfg->form == HWT_FORM_LANE || dev_opts->n == 0) {  // nothing of form 2 to choose
+    if (cfg->form == HWT_FORM_LANE || dev_opts->n == 0) {  // the caller fixed form 1: nothing to choose
         if (report) *report = rep;
         return HWT_OK;
     }
@@ -1218,6 +1218,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
             hwt_config c = base;
+            c.form = it->second.form;
             c.sort = it->second.sort;
             c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) |
                       (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT));
@@ -1270,6 +1271,14 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         if (v == best.narrow_per_cu) continue;
         hwt_config c = best;
         c.narrow_per_cu = v;
+        if ((rc = consider(c))) return rc;
+    }
+    // the inspector's last question: form 1 (one lane per option, columns in
+    // HBM) instead of form 2, for FP64 batches small enough that its column
+    // buffers (2.75 MB per 64 options) stay modest
+    if (base.precision == HWT_PREC_FP64 && dev_opts->n <= (1 << 18)) {
+        hwt_config c = best;
+        c.form = HWT_FORM_LANE;
         if ((rc = consider(c))) return rc;
     }
     *cfg = best;

@@ -160,7 +160,9 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
  * candidate settings of form 2 -- processing order of the wide trees (work,
  * height or width), narrow trees four or one per wavefront, the two pricing
  * kernels in sequence or side by side, and resident workgroups per CU of
- * each -- by greedy coordinate descent from *cfg, timing each whole pipeline
+ * each -- and then form 1 instead of form 2 for FP64 batches of at most
+ * 262,144 options (an inspector choosing the kernel form per batch), by
+ * greedy coordinate descent from *cfg (form 0 or 2), timing each whole pipeline
  * with HIP events (median of three), and writes the fastest back into *cfg.
  * The choice is cached per batch signature (precision, option count and the
  * tree-class counts and total work, each to a power of two) for the life of
