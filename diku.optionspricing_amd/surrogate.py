@@ -245,17 +245,24 @@ def calibrate_threshold(s: Surrogate, y, n: int = 250_000, seed: int = 77, targe
     for k in list(dev):
         if hasattr(dev[k], "free"):
             dev[k].free()
+    thr, accepted = threshold_from(tree, mean, spread, inbox, margin * target)
+    return {"threshold": thr, "calibration_options": n, "calibration_seed": seed,
+            "accepted_frac": accepted / n, "bound": margin * target}
+
+
+def threshold_from(tree, mean, spread, inbox, bound: float):
+    """(threshold, accepted count): the largest spread threshold at which every
+    in-box option with spread <= threshold has |mean - tree| <= bound.  Ties
+    at the threshold must all pass; with none passing, threshold 0 (an
+    option's spread can be exactly 0 only if all members agree to the bit)."""
     idx = np.nonzero(inbox)[0]
     order = idx[np.argsort(spread[idx], kind="stable")]
     run = np.maximum.accumulate(np.abs(mean[order] - tree[order]))
-    ok = np.nonzero(run <= margin * target)[0]
+    ok = np.nonzero(run <= bound)[0]
     k = int(ok[-1]) if ok.size and ok[0] == 0 else -1
-    # options tied at the chosen spread must all pass
     while k >= 0 and k + 1 < order.size and spread[order[k + 1]] == spread[order[k]]:
         k -= 1
-    thr = float(spread[order[k]]) if k >= 0 else 0.0
-    return {"threshold": thr, "calibration_options": n, "calibration_seed": seed,
-            "accepted_frac": (k + 1) / n, "bound": margin * target}
+    return (float(spread[order[k]]) if k >= 0 else 0.0), k + 1
 
 
 def with_threshold(s: Surrogate, threshold: float, infer_capacity: int, yield_capacity: int) -> Surrogate:

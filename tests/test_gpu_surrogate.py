@@ -212,3 +212,40 @@ def test_inference_kernel_matches_cpu_reference(setup):
     tol = R.RES * 4e-4 * np.maximum(1.0, np.abs(ys).max(axis=0)) + 1e-6 * np.abs(t) * R.RES + 1e-12
     np.testing.assert_array_less(np.abs(mean[rows] - want_mean), tol)
     np.testing.assert_array_less(np.abs(spread[rows] - want_spread), 2 * tol)
+
+
+@pytest.mark.parametrize("members", [1, 2])
+def test_smaller_ensembles_price_through_the_gate(pkg, setup, members):
+    """Ensembles of 1 and 2 members: a single member has spread 0, so every
+    in-box option takes the network's price; out-of-box ones still go to the
+    tree, and the network's prices equal predict()'s mean."""
+    y = setup["y"]
+    o = pkg.datasets.box_mix(51, 3000)
+    dev = pkg.capi.to_device(o, y)
+    s = pkg.surrogate.Surrogate(0, 4096, y.N, pkg.surrogate.config(members=members, spread_threshold=1e-3))
+    s.set_params(setup["s"].get_params()[:members])
+    plan = pkg.surrogate.fallback_plan(4096, y.N)
+    st = s.price(plan, dev)
+    got = dev["out"].to_host()
+    mean, spread, inbox = s.predict(dev)
+    if members == 1:
+        assert np.all(spread[inbox] == 0.0)
+        assert st["fallback"] == st["out_of_box"] == int((~inbox).sum())
+    accepted = inbox & (spread <= s.cfg.spread_threshold)
+    np.testing.assert_array_equal(got[accepted], mean[accepted])
+    plan.execute(dev)
+    plan.sync()
+    tree = dev["out"].to_host()
+    np.testing.assert_array_equal(got[~accepted], tree[~accepted])
+    s.close()
+    plan.close()
+
+
+def test_empty_batch_prices_nothing(pkg, setup):
+    y = setup["y"]
+    o = pkg.datasets.box(3, 0)
+    dev = pkg.capi.to_device(o, y)
+    plan = pkg.surrogate.fallback_plan(128, y.N)
+    st = setup["s"].price(plan, dev)
+    assert st["fallback"] == 0 and st["out_of_box"] == 0
+    plan.close()

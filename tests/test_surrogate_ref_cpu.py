@@ -105,3 +105,23 @@ def test_baseline_against_fine_tree(pkg, oracle):
         T, S = int(L / dt) * dt, int(200 * Tb) * dt
         cf = R.closed_form(X, T, S, a, s, typ, y)
         assert abs(tree - cf) < 5e-3, (tree, cf)
+
+
+def test_threshold_calibration_rule(pkg):
+    """The gate threshold rule on hand-made cases: the largest spread below
+    the first option (in spread order) whose error exceeds the bound, with
+    ties at the threshold all required to pass, out-of-box options ignored."""
+    f = pkg.surrogate.threshold_from
+    tree = np.zeros(6)
+    spread = np.array([1e-5, 2e-5, 3e-5, 3e-5, 5e-5, 1.0])
+    mean = np.array([1e-4, 2e-4, 3e-4, 9e-4, 1e-4, 0.0])
+    inbox = np.array([True, True, True, True, True, False])
+    # 4th option (spread 3e-5) fails a 5e-4 bound: its tie partner cannot be kept either
+    assert f(tree, mean, spread, inbox, 5e-4) == (2e-5, 2)
+    assert f(tree, mean, spread, inbox, 1e-3) == (5e-5, 5)
+    # the first option already fails: nothing is accepted
+    assert f(tree, mean, spread, inbox, 5e-5) == (0.0, 0)
+    # an out-of-box option with a huge error is ignored
+    mean2 = mean.copy()
+    mean2[5] = 10.0
+    assert f(tree, mean2, spread, inbox, 1e-3) == (5e-5, 5)
