@@ -837,6 +837,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     p->cols = lane ? (double *)(base + o_cols) : nullptr;
     if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
+    std::memset(p->h_tally, 0, T_NCOUNT * sizeof(int32_t));  // a plan never executed syncs clean
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
         hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
         hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||

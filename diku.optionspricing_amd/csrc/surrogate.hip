@@ -1359,7 +1359,7 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
     SURR_HIP(hipEventRecord(s->e2, st));
     SURR_HIP(hipMemcpyAsync(s->h_count, s->fb_count, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SURR_HIP(hipStreamSynchronize(st));
-    const int trc = hwt_plan_sync(tree, nullptr);
+    const int trc = o->n > 0 ? hwt_plan_sync(tree, nullptr) : HWT_OK;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float ms = 0.f;

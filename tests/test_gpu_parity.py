@@ -271,3 +271,11 @@ def test_autotune_chooses_and_caches(pkg, oracle):
     cfg2, rep2 = pkg.capi.autotune(dev)
     assert rep2["cached"] == 1 and rep2["candidates"] == 1
     assert cfg2.as_dict() == cfg.as_dict()
+
+
+def test_fresh_plan_syncs_clean(pkg):
+    """A plan that has never executed reports success and empty stats."""
+    plan = pkg.capi.Plan(1000, 10)
+    st = plan.sync()
+    assert st["tasks"] == 0 and st["node_updates"] == 0 and st["unsupported"] == 0
+    plan.close()
