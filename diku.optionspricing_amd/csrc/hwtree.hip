@@ -1165,11 +1165,12 @@ void tune_store_file(const std::string &key, const hwt_config &c)
       << c.wide_per_cu << ' ' << c.narrow_per_cu << '\n';
 }
 
-// Median pipeline time of three timed executions after one warm-up.
+// Median pipeline time of three timed executions after one warm-up, in the
+// mode the caller's plans will run in (graph replays unless HWT_FLAG_NO_GRAPH;
+// the warm-up captures the graph).
 int tune_time(const hwt_options_f64 *o, const hwt_yield_f64 *y, double *out, hwt_config c, double &ms,
               hwt_stats *first)
 {
-    c.flags |= HWT_FLAG_NO_GRAPH;
     hwt_plan *p = nullptr;
     int rc = hwt_plan_create(&p, o->n, y->n, &c);
     if (rc) return rc;

@@ -125,17 +125,24 @@ SOpts sopts(const hwt_options_f64 &o)
 // B~ = dt (1 - q^m) / (1 - q), q = e^{-a dt}, m = n - e (discrete bond
 // sensitivity); c0 = ln(100 P(S)/P(T)) minus the continuous-time convexity
 // terms B sigma^2 (1-e^{-aT})^2 / (2a^2) + B^2 sigma^2 (1 - e^{-2aT}) / (4a).
-// Whether option i lies inside the training box with an exercise step inside
-// its tree; c receives its tree constants.
-__device__ bool surr_in_box(const SOpts &o, int i, OptConst &c)
+// Whether option i's parameters lie inside the training box.
+__device__ bool surr_param_box(const SOpts &o, int i)
 {
     const double X = o.strike[i], Tb = o.maturity[i], L = o.length[i], a = o.a[i], sg = o.sigma[i];
     const int tu = o.tu[i], st = o.steps[i], ty = o.type[i];
-    const bool box = tu == 365 && (st == 12 || st == 36) && X >= 60.0 && X <= 100.0 && L >= 1.0 && L <= 30.0 &&
-                     Tb - L >= 1.0 && Tb - L <= 10.0 && a >= 0.01 && a <= 0.2 && sg >= 0.005 && sg <= 0.02 &&
-                     (ty == 0 || ty == 1);
-    if (!box) return false;
-    return option_constants(X, Tb, L, (uint16_t)tu, (uint16_t)st, a, sg, (int8_t)ty, 1 << 30, c) == ST_OK &&
+    return tu == 365 && (st == 12 || st == 36) && X >= 60.0 && X <= 100.0 && L >= 1.0 && L <= 30.0 &&
+           Tb - L >= 1.0 && Tb - L <= 10.0 && a >= 0.01 && a <= 0.2 && sg >= 0.005 && sg <= 0.02 &&
+           (ty == 0 || ty == 1);
+}
+
+// ... and its tree has the exercise step inside (always so for box parameters:
+// expiry >= 1 y puts e at 12 or more, bond >= expiry + 1 y puts n above e);
+// c receives the tree constants.
+__device__ bool surr_in_box(const SOpts &o, int i, OptConst &c)
+{
+    if (!surr_param_box(o, i)) return false;
+    return option_constants(o.strike[i], o.maturity[i], o.length[i], o.tu[i], o.steps[i], o.a[i], o.sigma[i],
+                            o.type[i], 1 << 30, c) == ST_OK &&
            c.e >= 1 && c.n > c.e;
 }
 
@@ -755,19 +762,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // Splits a batch before inference: options outside the training box go to
 // the fallback list (fb_idx, fb_count[0] and [1]), the others to the row list
-// of the inference kernel (rows, row_count).  One global atomic per block and
-// list; order inside the lists is arbitrary.
+// of the inference kernel (rows, row_count).  A block takes BOX_CHUNK options
+// and stages both lists in LDS, so the global counters see one atomic per
+// list per block (same-address atomics serialise in L2: one per 256 options
+// cost more than the whole test).  Order inside the lists is arbitrary.
+constexpr int BOX_CHUNK = 4096;
+
 __global__ __launch_bounds__(256) void k_box(SOpts o, int32_t *__restrict__ fb_idx, int32_t *__restrict__ fb_count,
                                              int32_t *__restrict__ rows, int32_t *__restrict__ row_count)
 {
     __shared__ int32_t nin, nout, bin, bout;
-    __shared__ int32_t lin[256], lout[256];
-    const int t = threadIdx.x, i = blockIdx.x * 256 + t;
+    __shared__ int32_t lin[BOX_CHUNK], lout[BOX_CHUNK];
+    const int t = threadIdx.x, i0 = blockIdx.x * BOX_CHUNK;
     if (t == 0) nin = nout = 0;
     __syncthreads();
-    if (i < o.n) {
-        OptConst c;
-        if (surr_in_box(o, i, c)) lin[atomicAdd(&nin, 1)] = i;
+    for (int i = i0 + t; i < min(i0 + BOX_CHUNK, o.n); i += 256) {
+        // the parameter test alone; a listed option that still fails the tree
+        // test is sent to the fallback list by the inference kernel
+        if (surr_param_box(o, i)) lin[atomicAdd(&nin, 1)] = i;
         else lout[atomicAdd(&nout, 1)] = i;
     }
     __syncthreads();
@@ -777,8 +789,8 @@ __global__ __launch_bounds__(256) void k_box(SOpts o, int32_t *__restrict__ fb_i
         if (nout) atomicAdd(&fb_count[1], nout);
     }
     __syncthreads();
-    if (t < nin) rows[bin + t] = lin[t];
-    if (t < nout) fb_idx[bout + t] = lout[t];
+    for (int k = t; k < nin; k += 256) rows[bin + k] = lin[k];
+    for (int k = t; k < nout; k += 256) fb_idx[bout + k] = lout[k];
 }
 
 // Fallback list -> compact option arrays, and prices back to input order.
@@ -1329,7 +1341,8 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
         // split: out-of-box options to the fallback list, in-box ones to the
         // inference rows (running the tree on the out-of-box part beside the
         // inference was measured slower: both compete for the same CUs)
-        k_box<<<blocks, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->inf_rows, s->fb_count + 4);
+        k_box<<<(o->n + BOX_CHUNK - 1) / BOX_CHUNK, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->inf_rows,
+                                                                s->fb_count + 4);
         // the ensemble on the in-box rows; options it is unsure of join the list
         InferArgs g{};
         g.o = sopts(*o);
