@@ -127,12 +127,8 @@ HWT_HD void edge_table_general(const TreeShape &t, double *tab)
 //   G00 = F00 - rho F01,  G10 = F10 - rho F11 + rho G00,  G11 = F11 + rho F01,
 //   G[a][a] = Faa - rhop Fab,  G[b][a] = Fba - rhop Fbb + rhop G[a][a],
 //   G[b][b] = Fbb + rhop Fab,  every other entry equals F.
-HWT_HD void edge_table(const TreeShape &t, double *tab)
+HWT_HD void edge_table_closed(const TreeShape &t, double *tab)
 {
-    if (t.w < 7) {
-        edge_table_general(t, tab);
-        return;
-    }
     const double M = t.M, rho = t.rho, rhop = t.rhop;
     const int J = t.jmax, a = t.w - 1, b = t.w - 2, c = t.w - 3, d = t.w - 4;
     const double E0 = node_disc(0, J, t.dr, t.dt), E1 = node_disc(1, J, t.dr, t.dt);
@@ -163,6 +159,12 @@ HWT_HD void edge_table(const TreeShape &t, double *tab)
         tab[i] = fwd[i];
         tab[18 + i] = bwd[i];
     }
+}
+
+HWT_HD void edge_table(const TreeShape &t, double *tab)
+{
+    if (t.w < 7) edge_table_general(t, tab);
+    else edge_table_closed(t, tab);
 }
 
 __device__ __forceinline__ void load_edge_table(const double *src, int lane, lds_f64 *tab)

@@ -1,12 +1,13 @@
 // hwtree.hip -- kernels and C ABI of the MI355X-native HW1F trinomial pricer.
 //
 // Pipeline of one execution (captured once into a HIP graph per plan):
-//   k_reset    clear queue head, histogram, tallies (a kernel, not a memset
-//              node, so graph replays are safe)
 //   k_setup    one thread per option: tree constants (the reference's
 //              OptionConstants), P(e dt) and P(n dt) from the yield curve, edge
 //              factors, work estimate; trivial options are priced right here
-//   k_scan     bucket offsets, largest work first (longest-processing-time order)
+//   k_scan     bucket offsets, largest work first (longest-processing-time
+//              order); it also clears every counter for the next execution,
+//              so no clearing kernel or memset node runs per execution
+//              (k_reset only runs for an empty batch, which has no k_scan)
 //   k_scatter  option indices into the task list
 //   k_price_wide / k_price_narrow  persistent kernels, launched side by side:
 //              each wavefront pulls options off its part of the task list and
@@ -88,7 +89,15 @@ enum Tally {
     T_WORKW = 10,   // int64, wide trees only
     T_NMID = 12,    // end of the mid-class tasks
     T_NPAD = 13,    // idle slots padding the row class
-    T_NCOUNT = 14
+    T_NCOUNT = 14,  // [0, T_NCOUNT) is what the host reads after an execution
+    // accumulated by k_setup's blocks, then moved into the slots above and
+    // cleared by k_scan
+    A_WORK = 16,    // int64
+    A_SEQWORK = 18, // int64
+    A_WORKW = 20,   // int64
+    A_INVALID = 22,
+    A_UNSUP = 23,
+    T_ALLOC = 32
 };
 
 __device__ __forceinline__ int log_bucket(int64_t key, int per_octave, int nb)
@@ -109,7 +118,7 @@ __device__ __forceinline__ int row_bucket(int w, int n)
 __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 {
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i < T_NCOUNT) tally[i] = 0;
+    if (i < T_ALLOC) tally[i] = 0;
     if (i < NBUCKET) {
         hist[i] = 0;
         cursor[i] = 0;
@@ -122,6 +131,99 @@ __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 // counters hit by every option serialise in L2).
 constexpr int SETUP_BLOCK = 256;
 constexpr int SETUP_YMAX = 2048;  // knots staged in LDS by k_setup (24 KB)
+constexpr int SCAN_PER = NBUCKET / SETUP_BLOCK;
+static_assert(NBUCKET % SETUP_BLOCK == 0, "scan positions per thread");
+
+// Bucket at scan position p: the wide class, then the mid class, each largest
+// key first unless an ascending order was asked for, then the row class by
+// decreasing (K, height).
+__device__ __forceinline__ int scan_bucket(int p, int asc)
+{
+    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID;
+    if (p < NW) return asc ? B_WIDE + p : NBUCKET - 1 - p;
+    if (p < NW + NM) return asc ? B_MID + (p - NW) : B_WIDE - 1 - (p - NW);
+    return NBUCKET - 1 - p;
+}
+
+// One block of SETUP_BLOCK threads, after k_setup: exclusive offsets of the
+// buckets in scan order, the class boundaries and counts into the
+// host-visible tally, and every counter cleared for the next execution
+// (histogram, scatter cursors, queue heads, accumulators).  Each K part of the
+// row class is padded to a multiple of four tasks with idle slots (order -1),
+// so every group of four row tasks shares one K and an option's arithmetic
+// never depends on which options share its wave (not for the lane form,
+// which has no row groups).
+__global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist, int32_t *__restrict__ offs,
+                                                      int32_t *__restrict__ cursor, int32_t *__restrict__ order,
+                                                      int32_t *__restrict__ tally, int32_t asc, int32_t pad_rows)
+{
+    const int tid = threadIdx.x;
+    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NKB = B_MID / ROW_MAXK;
+    __shared__ int32_t ktot[ROW_MAXK];
+    __shared__ int32_t part[SETUP_BLOCK];
+    __shared__ int32_t npad;
+    int idx[SCAN_PER], cnt[SCAN_PER];
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        idx[q] = scan_bucket(tid * SCAN_PER + q, asc);
+        cnt[q] = hist[idx[q]];
+    }
+    if (tid < ROW_MAXK) ktot[tid] = 0;
+    if (tid == 0) npad = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        hist[idx[q]] = 0;
+        cursor[idx[q]] = 0;
+        if (idx[q] < B_MID && cnt[q]) atomicAdd(&ktot[idx[q] / NKB], cnt[q]);
+    }
+    __syncthreads();
+    // the lowest bucket of each K part comes last in the scan and takes the pad
+    int v[SCAN_PER], pad[SCAN_PER], sum = 0, pads = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        pad[q] = (pad_rows && idx[q] < B_MID && idx[q] % NKB == 0) ? (ROWS - ktot[idx[q] / NKB] % ROWS) % ROWS : 0;
+        v[q] = cnt[q] + pad[q];
+        sum += v[q];
+        pads += pad[q];
+    }
+    part[tid] = sum;
+    if (pads) atomicAdd(&npad, pads);
+    __syncthreads();
+    for (int off = 1; off < SETUP_BLOCK; off <<= 1) {
+        const int32_t add = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        const int p = tid * SCAN_PER + q;
+        offs[idx[q]] = run;
+        run += v[q];
+        for (int j = 0; j < pad[q]; ++j) order[run - pad[q] + j] = -1;
+        if (p == NW - 1) tally[T_NLARGE] = run;
+        if (p == NW + NM - 1) tally[T_NMID] = run;
+        if (p == NBUCKET - 1) tally[T_NTASK] = run;
+    }
+    if (tid == 0) {
+        tally[T_NPAD] = npad;
+        tally[T_QUEUE] = 0;
+        tally[T_QUEUE_S] = 0;
+        int64_t *t64 = reinterpret_cast<int64_t *>(tally);
+        t64[T_WORK / 2] = t64[A_WORK / 2];
+        t64[T_SEQWORK / 2] = t64[A_SEQWORK / 2];
+        t64[T_WORKW / 2] = t64[A_WORKW / 2];
+        tally[T_INVALID] = tally[A_INVALID];
+        tally[T_UNSUP] = tally[A_UNSUP];
+        t64[A_WORK / 2] = 0;
+        t64[A_SEQWORK / 2] = 0;
+        t64[A_WORKW / 2] = 0;
+        tally[A_INVALID] = 0;
+        tally[A_UNSUP] = 0;
+    }
+}
 
 __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
                                                        double unit_dmin, int32_t rows,
@@ -135,6 +237,24 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     __shared__ double yr[SETUP_YMAX];       // the yield curve, staged once per block
     __shared__ int32_t yd[SETUP_YMAX];
     const int tid = threadIdx.x;
+    const int i = tid + blockIdx.x * SETUP_BLOCK;
+    const int nopt = o.count ? min(*o.count, o.n) : o.n;
+    const bool mine = i < nopt;
+    // the option's fields are requested before the curve is staged, so the
+    // two loads overlap
+    double strike = 0.0, mat = 0.0, len = 0.0, a = 0.0, sigma = 0.0;
+    uint16_t tu = 0, ts = 0;
+    int8_t type = 0;
+    if (mine) {
+        strike = o.strike[i];
+        mat = o.maturity[i];
+        len = o.length[i];
+        tu = o.term_unit[i];
+        ts = o.term_steps[i];
+        a = o.a[i];
+        sigma = o.sigma[i];
+        type = o.type[i];
+    }
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lh[k] = 0;
     if (tid < 3) lsum[tid] = 0ull;
     if (tid < 2) lbad[tid] = 0;
@@ -154,12 +274,9 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     auto yield = [&](double t, int tu) {
         return sorted ? yield_at_sorted(t, tu, yr, yd, y.n) : yield_at(t, tu, y.rate, y.day, y.n);
     };
-    const int i = tid + blockIdx.x * SETUP_BLOCK;
-    const int nopt = o.count ? min(*o.count, o.n) : o.n;
-    if (i < nopt) {
+    if (mine) {
         OptConst c;
-        const int32_t st = option_constants(o.strike[i], o.maturity[i], o.length[i], o.term_unit[i],
-                                            o.term_steps[i], o.a[i], o.sigma[i], o.type[i], maxWidth, c);
+        const int32_t st = option_constants(strike, mat, len, tu, ts, a, sigma, type, maxWidth, c);
         c.status = st;
         int b = -1;
         if (st == ST_INVALID) {
@@ -171,7 +288,6 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
         } else if (c.n == 0) {
             out[i] = 100.0;  // no steps: the face value (seq/Seq.hpp:119,171)
         } else {
-            const int tu = o.term_unit[i];
             const double tn = c.n * c.dt;
             c.Pn = exp(-yield(tn, tu) * tn);
             if (c.e < 0) {
@@ -183,8 +299,20 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 edge_factors(t);
                 c.rho = t.rho;
                 c.rhop = t.rhop;
-                edge_table(t, edge + (size_t)i * EDGE_TAB);
-                c.unit = unit_form_ok(t, edge + (size_t)i * EDGE_TAB, unit_dmin) ? 1 : 0;
+                // trees of 7 or more nodes (all but the tiniest) build the
+                // table in registers, store it, and check it from the registers
+                double *eo = edge + (size_t)i * EDGE_TAB;
+                if (t.w >= 7) {
+                    double tab[EDGE_TAB];
+                    edge_table_closed(t, tab);
+#pragma unroll
+                    for (int k = 0; k < EDGE_TAB / 2; ++k)
+                        reinterpret_cast<double2 *>(eo)[k] = make_double2(tab[2 * k], tab[2 * k + 1]);
+                    c.unit = unit_form_ok(t, tab, unit_dmin) ? 1 : 0;
+                } else {
+                    edge_table_general(t, eo);
+                    c.unit = unit_form_ok(t, eo, unit_dmin) ? 1 : 0;
+                }
                 c.work = dual_node_updates(c.n, c.e, c.jmax);
                 const int64_t seqw = seq_node_updates(c.n, c.jmax);
                 atomicAdd(&lsum[0], (unsigned long long)c.work);
@@ -211,55 +339,12 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
         if (lh[k]) atomicAdd(&hist[k], lh[k]);
     if (tid == 0) {
-        if (lsum[0]) atomicAdd((unsigned long long *)&tally[T_WORK], lsum[0]);
-        if (lsum[1]) atomicAdd((unsigned long long *)&tally[T_SEQWORK], lsum[1]);
-        if (lsum[2]) atomicAdd((unsigned long long *)&tally[T_WORKW], lsum[2]);
-        if (lbad[0]) atomicAdd(&tally[T_INVALID], lbad[0]);
-        if (lbad[1]) atomicAdd(&tally[T_UNSUP], lbad[1]);
+        if (lsum[0]) atomicAdd((unsigned long long *)&tally[A_WORK], lsum[0]);
+        if (lsum[1]) atomicAdd((unsigned long long *)&tally[A_SEQWORK], lsum[1]);
+        if (lsum[2]) atomicAdd((unsigned long long *)&tally[A_WORKW], lsum[2]);
+        if (lbad[0]) atomicAdd(&tally[A_INVALID], lbad[0]);
+        if (lbad[1]) atomicAdd(&tally[A_UNSUP], lbad[1]);
     }
-}
-
-// Exclusive offsets of the buckets (one thread per bucket): the wide class,
-// then the mid class, each largest key first unless an ascending order was
-// asked for, then the row class by decreasing (K, height).  Each K part of
-// the row class is padded to a multiple of four tasks with idle slots
-// (order -1), so every group of four row tasks shares one K and an option's
-// arithmetic never depends on which options share its wave (not for the
-// lane form, which has no row groups).
-__global__ __launch_bounds__(NBUCKET) void k_scan(const int32_t *__restrict__ hist, int32_t *__restrict__ offs,
-                                                  int32_t *__restrict__ tally, int32_t *__restrict__ order,
-                                                  int32_t asc, int32_t pad_rows)
-{
-    __shared__ int32_t s[NBUCKET];
-    __shared__ int32_t ktot[ROW_MAXK];
-    const int tid = threadIdx.x;
-    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NKB = B_MID / ROW_MAXK;
-    int idx;
-    if (tid < NW) idx = asc ? B_WIDE + tid : NBUCKET - 1 - tid;
-    else if (tid < NW + NM) idx = asc ? B_MID + (tid - NW) : B_WIDE - 1 - (tid - NW);
-    else idx = NBUCKET - 1 - tid;
-    const int32_t count = hist[idx];
-    if (tid < ROW_MAXK) ktot[tid] = 0;
-    __syncthreads();
-    if (idx < B_MID && count) atomicAdd(&ktot[idx / NKB], count);
-    __syncthreads();
-    // the lowest bucket of each K part comes last in the scan and takes the pad
-    const int pad = (pad_rows && idx < B_MID && idx % NKB == 0) ? (ROWS - ktot[idx / NKB] % ROWS) % ROWS : 0;
-    const int32_t v = count + pad;
-    s[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < NBUCKET; off <<= 1) {
-        const int32_t add = tid >= off ? s[tid - off] : 0;
-        __syncthreads();
-        s[tid] += add;
-        __syncthreads();
-    }
-    offs[idx] = s[tid] - v;
-    for (int j = 0; j < pad; ++j) order[s[tid] - pad + j] = -1;
-    if (pad) atomicAdd(&tally[T_NPAD], pad);
-    if (tid == NW - 1) tally[T_NLARGE] = s[tid];
-    if (tid == NW + NM - 1) tally[T_NMID] = s[tid];
-    if (tid == NBUCKET - 1) tally[T_NTASK] = s[tid];
 }
 
 // Option indices into the task list, bucket by bucket.  Ranks inside a block
@@ -662,15 +747,17 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     p->timed_split = p->timed_kernel && p->cfg.form != HWT_FORM_LANE;
     const int sort = p->cfg.sort ? p->cfg.sort : 'L';
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
-    k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
+    // an empty batch launches no k_setup, whose last block would otherwise
+    // leave the counters of this execution
+    if (n == 0) k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
         const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
         const bool rows = p->cfg.form != HWT_FORM_LANE && !(p->cfg.flags & HWT_FLAG_NO_ROWS);
+        // the lane form prices tasks one per lane: no row groups, no pads
         k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, unit_dmin, rows, p->cst, p->edge,
                                                p->bucket, p->hist, p->tally, out);
-        // the lane form prices tasks one per lane: no row groups, no pads
-        k_scan<<<1, NBUCKET, 0, s>>>(p->hist, p->offs, p->tally, p->order, asc, rows);
+        k_scan<<<1, SETUP_BLOCK, 0, s>>>(p->hist, p->offs, p->cursor, p->order, p->tally, asc, rows);
         k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
@@ -813,7 +900,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
     const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
-    const size_t o_tally = carve(T_NCOUNT * sizeof(int32_t));
+    const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));
     const size_t o_scw = carve((size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
     const bool lane = p->cfg.form == HWT_FORM_LANE;
     if (lane) {
@@ -835,6 +922,12 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     p->tally = (int32_t *)(base + o_tally);
     p->scratch_wide = (double *)(base + o_scw);
     p->cols = lane ? (double *)(base + o_cols) : nullptr;
+    // counters start clear; from then on every execution leaves them clear
+    if (hipMemsetAsync(p->hist, 0, NBUCKET * sizeof(int32_t), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->cursor, 0, NBUCKET * sizeof(int32_t), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->tally, 0, T_ALLOC * sizeof(int32_t), p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+        return bail(HWT_EHIP, "clearing the plan's counters failed");
     if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     std::memset(p->h_tally, 0, T_NCOUNT * sizeof(int32_t));  // a plan never executed syncs clean
