@@ -163,8 +163,9 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
  * each -- and then form 1 instead of form 2 for FP64 batches of at most
  * 262,144 options (an inspector choosing the kernel form per batch), by
  * greedy coordinate descent from *cfg (form 0 or 2), timing each whole pipeline
- * with HIP events (median of three, replaying the plan's graph unless *cfg has
- * HWT_FLAG_NO_GRAPH), and writes the fastest back into *cfg.
+ * with HIP events (median of seven after five warm-ups, replaying the plan's
+ * graph unless *cfg has HWT_FLAG_NO_GRAPH), and writes the fastest back into
+ * *cfg.
  * The choice is cached per batch signature (precision, option count and the
  * tree-class counts and total work, each to a power of two) for the life of
  * the process, and appended to the file named by the environment variable
