@@ -553,22 +553,26 @@ struct Window {
     int K, base;
 };
 
-// Smallest centred window of half width 32K that can hold half-width D, or
-// the full window when none fits inside the tree.
+// Smallest centred window of half width 32K, K a multiple of G, that can hold
+// half-width D, or the full window when none fits inside the tree.  Windows
+// grow (forward) and shrink (backward) G nodes per lane at a time: a larger G
+// steps a few padding nodes more but changes window -- and pays the
+// coefficient set-up of a phase -- G times less often.
+template <int G>
 __device__ __forceinline__ Window window_for(int D, const TreeShape &t)
 {
-    const int Kg = D / 32 + 1;
+    const int Kg = (D / 32 / G + 1) * G;
     if (32 * Kg <= t.jmax) return Window{Kg, t.jmax - 32 * Kg};
     return Window{(t.w + WAVE - 1) / WAVE, 0};
 }
 
 // Forward pass: state 0 (delta at the root) -> state e.  The state ends up in
 // out[lo, hi) (global scratch); tree nodes outside that range are zero.
-template <class T, int KMAX>
+template <class T, int KMAX, int G>
 __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, typename Lds<T>::type *lds,
                                              const lds_f64 *etab, T *out, int &lo, int &hi, bool unit)
 {
-    Window win = window_for(0, t);
+    Window win = window_for<G>(0, t);
     int in_lo = 0, in_hi = 0;  // empty: initial values
     int i = 0;
     for (;;) {
@@ -581,7 +585,7 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
         if (last) break;
-        win = window_for(i + 1, t);
+        win = window_for<G>(i + 1, t);
     }
     lo = in_lo;
     hi = in_hi;
@@ -590,25 +594,25 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
 // Backward pass: state n (bond face value 100) -> state e, keeping only the
 // nodes that can still reach the exercise nodes |j| <= min(e, jmax).  The
 // state ends up in lds[lo, hi).
-template <class T, int KMAX>
+template <class T, int KMAX, int G>
 __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, typename Lds<T>::type *lds,
                                               const lds_f64 *etab, int &lo, int &hi, bool unit)
 {
     const int ep = min(e, t.jmax);
-    const int Kgmax = t.jmax / 32;
+    const int Kgmax = t.jmax / 32 / G * G;  // the largest window below the full one (0: none)
     int s = n;
-    Window win = window_for(min(t.jmax, ep + (s - e)), t);
+    Window win = window_for<G>(min(t.jmax, ep + (s - e)), t);
     int in_lo = 0, in_hi = 0;
     while (s > e) {
         const bool full = 32 * win.K > t.jmax;
-        const int klo = full ? Kgmax : win.K - 1;
+        const int klo = full ? Kgmax : win.K - G;  // the next window down
         const int send = max(e, e + 32 * klo - 1 - ep);
         dispatch_phase<T, KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, (T *)nullptr,
                                        unit);
         s = send;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
-        if (s > e) win = window_for(min(t.jmax, ep + (s - e)), t);
+        if (s > e) win = window_for<G>(min(t.jmax, ep + (s - e)), t);
     }
     lo = in_lo;
     hi = in_hi;
