@@ -135,6 +135,36 @@ def test_widest_supported_tree(pkg, oracle):
     assert_parity(pkg.capi.price(o, y), oracle.price(o, y))
 
 
+def test_window_phase_boundaries(pkg, oracle):
+    """Wide trees whose register window changes exactly where the light cone
+    reaches a window edge: half-widths J at and around multiples of 128 nodes
+    (a window grows four nodes per lane, 128 nodes of half-width, at a time),
+    exercise steps at and around 32-step multiples and at J, and heights that
+    end the backward pass just before, at and after a shrink."""
+    y = pkg.datasets.synthetic_yield()
+    dt = 1 / 36
+    cols = {k: [] for k in ("strike", "maturity", "length", "term_unit", "term_steps", "a", "sigma", "type")}
+    for J in (127, 128, 129, 255, 256, 257, 383, 384, 385, 511, 512, 513, 663):
+        a = -math.log(1 - 0.184 / (J - 0.5)) / dt
+        for e in sorted({0, 1, 31, 32, 127, 128, 129, 255, 256, 257, J - 1, J, J + 1}):
+            for extra in (1, 2, 129, 259, 2 * J + 7):
+                n = e + extra
+                cols["strike"].append(70.0 + (J + e + extra) % 30)
+                cols["maturity"].append((n + 0.5) * dt)
+                cols["length"].append((e + 0.5) * dt)
+                cols["term_unit"].append(365)
+                cols["term_steps"].append(36)
+                cols["a"].append(a)
+                cols["sigma"].append(0.01)
+                cols["type"].append((J + e + extra) % 2)
+    o = pkg.Options(**cols)
+    assert {oracle.constants(o, i).width for i in range(o.N)} >= {255, 257, 259, 511, 1027, 1327}
+    ref = oracle.price(o, y)
+    assert_parity(pkg.capi.price(o, y), ref, "window phase boundaries")
+    assert_parity(pkg.capi.price(o, y, pkg.capi.config(flags=pkg.capi.FLAG_NO_GRAPH, sort="-")), ref,
+                  "window phase boundaries, unsorted")
+
+
 def test_too_wide_is_unsupported_and_others_priced(pkg, oracle):
     y = pkg.datasets.synthetic_yield()
     dt = 1 / 36
