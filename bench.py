@@ -116,6 +116,7 @@ def main():
     ap.add_argument("--sort", default="L", help="--profile: processing order of the wide trees")
     ap.add_argument("--concurrent", action="store_true", help="--profile: narrow-tree kernel beside the wide one")
     ap.add_argument("--no-rows", action="store_true", help="--profile: narrow trees one per wavefront")
+    ap.add_argument("--fused", action="store_true", help="--profile: one persistent kernel for every tree")
     ap.add_argument("--profile-surrogate", action="store_true",
                     help="profiler run of the surrogate: a short training (2 epochs on 262,144 tree-priced "
                          "samples), then warmup + steps inference-with-fallback passes over the 1M-option mix")
@@ -152,7 +153,7 @@ def main():
         dm._ck(dm.hip().hipMemcpy(dst.ptr, src.ptr, n, dm.D2D))
         dm.synchronize()
         flags = capi.FLAG_NO_GRAPH | (capi.FLAG_CONCURRENT if args.concurrent else 0) | \
-            (capi.FLAG_NO_ROWS if args.no_rows else 0)
+            (capi.FLAG_NO_ROWS if args.no_rows else 0) | (capi.FLAG_FUSED if args.fused else 0)
         plan = capi.Plan(opts.N, y.N, capi.config(sort=args.sort, flags=flags))
         wide = []
         for i in range(args.warmup + args.steps):

@@ -93,6 +93,7 @@ enum Tally {
     T_NMID = 12,    // end of the mid-class tasks
     T_NPAD = 13,    // idle slots padding the row class
     T_NCOUNT = 14,  // [0, T_NCOUNT) is what the host reads after an execution
+    T_QUEUE_R = 15, // next chunk of row groups (fused kernel)
     // accumulated by k_setup's blocks, then moved into the slots above and
     // cleared by k_scan
     A_WORK = 16,    // int64
@@ -214,6 +215,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
         tally[T_NPAD] = npad;
         tally[T_QUEUE] = 0;
         tally[T_QUEUE_S] = 0;
+        tally[T_QUEUE_R] = 0;
         int64_t *t64 = reinterpret_cast<int64_t *>(tally);
         t64[T_WORK / 2] = t64[A_WORK / 2];
         t64[T_SEQWORK / 2] = t64[A_SEQWORK / 2];
@@ -543,17 +545,31 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
 // group per wave at a time (static stride); the last group may hold fewer
 // options, and its idle rows repeat the group's first option without
 // writing a price.
-template <class T>
+//   DYNAMIC: groups are claimed ROW_CHUNK at a time from tally[T_QUEUE_R]
+//   (the fused kernel, whose wavefronts reach the row class at different
+//   times); otherwise wave b takes groups b, b + grid, ...
+constexpr int ROW_CHUNK = 4;
+template <class T, bool DYNAMIC>
 __device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                                  const int32_t *__restrict__ order, int first, int last,
-                                                 double *__restrict__ out)
+                                                 double *__restrict__ out, int32_t *__restrict__ tally)
 {
     __shared__ double rowbuf[2 * ROWS * ROW_MAX_WIDTH];
     const int lane = threadIdx.x, row = lane / ROW, l = lane % ROW;
     lds_f64 *ybuf = (lds_f64 *)rowbuf + row * ROW_MAX_WIDTH;
     lds_f64 *bbuf = (lds_f64 *)rowbuf + (ROWS + row) * ROW_MAX_WIDTH;
     const int ngroups = (last - first + ROWS - 1) / ROWS;
-    for (int g = uni((int)blockIdx.x); g < ngroups; g += uni((int)gridDim.x)) {
+    int g, gend;
+    auto claim = [&]() {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&tally[T_QUEUE_R], 1);
+        c = uni(__shfl(c, 0, WAVE));
+        g = c * ROW_CHUNK;
+        gend = min(g + ROW_CHUNK, ngroups);
+    };
+    if (DYNAMIC) claim();
+    else g = uni((int)blockIdx.x);
+    while (g < ngroups) {
         const int base = first + g * ROWS;
         const int mine = base + row < last ? order[base + row] : -1;
         const bool active = mine >= 0;
@@ -585,6 +601,11 @@ __device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cs
         default: p = price_rows<T, 4>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
         }
         if (active && l == 0) out[o] = p;
+        if (DYNAMIC) {
+            if (++g >= gend) claim();
+        } else {
+            g += uni((int)gridDim.x);
+        }
     }
 }
 
@@ -595,7 +616,22 @@ __global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restric
 {
     const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
     price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr);
-    price_row_groups<T>(cst, edge, order, nmid, ntask, out);
+    price_row_groups<T, false>(cst, edge, order, nmid, ntask, out, tally);
+}
+
+// Every class in one persistent kernel: the wide trees from the LPT queue,
+// then the mid class and the row groups from their own queues, so waves that
+// run out of wide trees early price the narrow ones while the last wide
+// trees finish.
+template <class T>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void k_price_fused(
+    const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
+    int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
+{
+    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
+    price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
+    price_tasks<T, KSMALL, true, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr);
+    price_row_groups<T, true>(cst, edge, order, nmid, ntask, out, tally);
 }
 
 // Form 1: one lane per option over every scheduled task (tasks ordered by
@@ -717,7 +753,7 @@ struct hwt_plan {
     hipStream_t stream = nullptr;
     int32_t capacity = 0, ycap = 0;
     hwt_config cfg{};
-    int grid_wide = 0, grid_narrow = 0;
+    int grid_wide = 0, grid_narrow = 0, grid_fused = 0;
     hipStream_t side = nullptr;       // narrow-tree kernel runs here, concurrently
     hipEvent_t fork = nullptr, join = nullptr;
     double *scratch_wide = nullptr;
@@ -771,6 +807,23 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
             const int lb = p->lane_block;
             k_price_lane<<<(n + lb - 1) / lb, lb, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->cols);
             if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
+            HWT_HIP(hipGetLastError());
+            return HWT_OK;
+        }
+        if (p->cfg.flags & HWT_FLAG_FUSED) {
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
+            if (p->cfg.precision == HWT_PREC_MIXED)
+                k_price_fused<float><<<p->grid_fused, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                    p->scratch_wide);
+            else
+                k_price_fused<double><<<p->grid_fused, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                     p->scratch_wide);
+            if (time_kernel) {
+                HWT_HIP(hipEventRecord(p->evw1, s));
+                HWT_HIP(hipEventRecord(p->evn0, s));  // no separate narrow kernel: narrow_ms ~ 0
+                HWT_HIP(hipEventRecord(p->evn1, s));
+                HWT_HIP(hipEventRecord(p->evk1, s));
+            }
             HWT_HIP(hipGetLastError());
             return HWT_OK;
         }
@@ -890,6 +943,15 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     if (p->cfg.narrow_per_cu > 0) per_cu_n = std::min(per_cu_n, p->cfg.narrow_per_cu);
     p->grid_wide = per_cu_w * cus;
     p->grid_narrow = per_cu_n * cus;
+    // the fused kernel shares the wide kernel's scratch slots, so its grid
+    // never exceeds the wide kernel's
+    int per_cu_f = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu_f, mixed ? (const void *)k_price_fused<float> : (const void *)k_price_fused<double>, WAVE, 0) !=
+            hipSuccess ||
+        per_cu_f < 1)
+        per_cu_f = per_cu_w;
+    p->grid_fused = std::min(per_cu_f, per_cu_w) * cus;
     if (const char *v = getenv("HWT_VERBOSE"); v && *v && *v != '0')
         fprintf(stderr, "hwtree plan: device %d, %d CUs, wide kernel %d blocks (%d/CU), narrow kernel %d blocks (%d/CU)\n",
                 p->device, cus, p->grid_wide, per_cu_w, p->grid_narrow, per_cu_n);
@@ -1264,7 +1326,8 @@ void tune_store_file(const std::string &key, const hwt_config &c)
     const char *path = getenv("HWT_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << ' ' << c.form << ' ' << c.sort << ' ' << (c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) << ' '
+    const int kept = c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED);
+    f << key << ' ' << c.form << ' ' << c.sort << ' ' << kept << ' '
       << c.wide_per_cu << ' ' << c.narrow_per_cu << '\n';
 }
 
@@ -1345,8 +1408,8 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
             hwt_config c = base;
             c.form = it->second.form;
             c.sort = it->second.sort;
-            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT)) |
-                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT));
+            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED)) |
+                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED));
             c.wide_per_cu = it->second.wide_per_cu;
             c.narrow_per_cu = it->second.narrow_per_cu;
             *cfg = c;
@@ -1381,7 +1444,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         c.sort = so;
         if ((rc = consider(c))) return rc;
     }
-    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_CONCURRENT}) {
+    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_CONCURRENT, HWT_FLAG_FUSED}) {
         hwt_config c = best;
         c.flags ^= bit;
         if ((rc = consider(c))) return rc;

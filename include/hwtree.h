@@ -104,6 +104,10 @@ typedef struct {
 /* Form 2: run the narrow-tree kernel beside the wide-tree kernel on a second
  * stream instead of after it on the same stream. */
 #define HWT_FLAG_CONCURRENT 4
+/* Form 2: one persistent kernel for every tree: its wavefronts take the
+ * wide trees first, then claim the narrow ones in chunks as they run out, so
+ * narrow trees fill the wide trees' tail.  Overrides HWT_FLAG_CONCURRENT. */
+#define HWT_FLAG_FUSED 8
 
 typedef struct {
     double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),
@@ -159,8 +163,8 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
  * and cuda-multi/Kernel.cuh:374).  Prices the device batch repeatedly with
  * candidate settings of form 2 -- processing order of the wide trees (work,
  * height or width), narrow trees four or one per wavefront, the two pricing
- * kernels in sequence or side by side, and resident workgroups per CU of
- * each -- and then form 1 instead of form 2 for FP64 batches of at most
+ * kernels in sequence, side by side or fused, and resident workgroups per CU
+ * of each -- and then form 1 instead of form 2 for FP64 batches of at most
  * 262,144 options (an inspector choosing the kernel form per batch), by
  * greedy coordinate descent from *cfg (form 0 or 2), timing each whole pipeline
  * with HIP events (median of seven after five warm-ups, replaying the plan's

@@ -273,7 +273,8 @@ def test_yield_curve_shapes(pkg, oracle, curve):
     assert_parity(pkg.capi.price(opts, y), oracle.price(opts, y), curve)
 
 
-@pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"wide_per_cu": 2, "narrow_per_cu": 4},
+@pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"flags": 8}, {"flags": 10},
+                                     {"flags": 9}, {"wide_per_cu": 2, "narrow_per_cu": 4},
                                      {"sort": "H"}, {"sort": "W"}])
 def test_form2_settings_price_the_same(pkg, oracle, setting):
     """Every setting the autotuner can choose prices within the parity bar."""
