@@ -200,7 +200,10 @@ def main():
     # HIP events around the wide-tree kernel on the stream it runs on
     cfg_k = capi.config()
     C.memmove(C.byref(cfg_k), C.byref(tuned), C.sizeof(cfg_k))
-    cfg_k.flags |= capi.FLAG_NO_GRAPH
+    # the wide-tree kernel on its own: serial arrangement whatever the tuner
+    # chose (fused, it would include the narrow trees; side by side, it would
+    # share the CUs with them)
+    cfg_k.flags = (cfg_k.flags & ~(capi.FLAG_FUSED | capi.FLAG_CONCURRENT)) | capi.FLAG_NO_GRAPH
     plan_k = capi.Plan(opts.N, y.N, cfg_k)
     wide, narrow, kern = [], [], []
     for i in range(args.warmup + args.steps):
