@@ -243,6 +243,46 @@ def test_plan_graph_replays(pkg, oracle):
     plan2.close()
 
 
+@pytest.mark.parametrize("flags", [0, 1], ids=["graph", "eager"])
+def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
+    """One plan, executions queued back to back with no sync between them,
+    then an empty batch, a batch with an invalid option and a good batch
+    again: each execution starts from clear counters (the scan kernel clears
+    them for the next one), so prices and the tallies read at sync are the
+    last execution's own."""
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=6, n=1500)
+    dev = pkg.capi.to_device(opts, y)
+    ref = oracle.price(opts, y)
+    plan = pkg.capi.Plan(opts.N + 1, y.N, pkg.capi.config(flags=flags))
+    plan.execute(dev)
+    st1 = plan.sync()
+    assert_parity(dev["out"].to_host(), ref, "first execution")
+    keys = ("tasks", "wide_tasks", "node_updates", "wide_node_updates", "seq_node_updates", "unsupported")
+    assert st1["tasks"] == opts.N and st1["wide_tasks"] > 0
+    for _ in range(3):
+        plan.execute(dev)
+    st3 = plan.sync()
+    assert_parity(dev["out"].to_host(), ref, "after three back-to-back executions")
+    assert {k: st3[k] for k in keys} == {k: st1[k] for k in keys}
+    plan.execute(dev, n=0)
+    st0 = plan.sync()
+    assert st0["tasks"] == 0 and st0["node_updates"] == 0 and st0["wide_tasks"] == 0
+    bad = pkg.Options.concat([opts, _one(pkg, a=0.0)])
+    dbad = pkg.capi.to_device(bad, y)
+    plan.execute(dbad)
+    with pytest.raises(pkg.capi.HwtError) as err:
+        plan.sync()
+    assert err.value.code == pkg.capi.HWT_EINVAL
+    assert_parity(dbad["out"].to_host()[:opts.N], ref, "good options beside an invalid one")
+    dev["out"].fill_zero()
+    plan.execute(dev)
+    st = plan.sync()  # no invalid count left over from the previous execution
+    assert_parity(dev["out"].to_host(), ref, "after the invalid batch")
+    assert {k: st[k] for k in keys} == {k: st1[k] for k in keys}
+    plan.close()
+
+
 def test_plan_capacity_enforced(pkg):
     y = pkg.datasets.synthetic_yield()
     opts = pkg.datasets.uniform_1k()
