@@ -119,6 +119,7 @@ __device__ __forceinline__ int row_bucket(int w, int n)
     return (K - 1) * 160 + h;
 }
 
+// Every counter and tally to zero: the empty batch's whole pipeline.
 __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 {
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
@@ -793,8 +794,8 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     p->timed_split = p->timed_kernel && p->cfg.form != HWT_FORM_LANE;
     const int sort = p->cfg.sort ? p->cfg.sort : 'L';
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
-    // an empty batch launches no k_setup, whose last block would otherwise
-    // leave the counters of this execution
+    // an empty batch launches no k_setup and no k_scan: the tallies it
+    // reports (all zero) come from a clearing kernel instead
     if (n == 0) k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
