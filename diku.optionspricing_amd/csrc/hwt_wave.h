@@ -397,10 +397,11 @@ __device__ __forceinline__ void unit_out(const TreeShape &t, int base, int lane,
     }
 }
 
-// Largest window the unit form is used for.  FP64: beyond 12 nodes per lane
-// the transform's coefficient setup no longer fits the register budget of two
-// waves per SIMD; windows of K <= 12 carry about 80% of the stencil work of a
-// typical wide batch.  FP32 (mixed mode): not used -- the undiscounted
+// Largest window the unit form is used for.  FP64: 12 nodes per lane.
+// Windows of K <= 12 carry about 80% of the stencil work of a typical wide
+// batch; limits of 14 and 16 still fit two waves per SIMD with a few more
+// spilled registers but run no faster, and from 18 on the spills grow into
+// the hundreds.  FP32 (mixed mode): not used -- the undiscounted
 // backward state of a long, weakly mean-reverting bond already comes within a
 // few powers of ten of the FP32 limit (about 1e34 for a 100-year bond with
 // a = 0.009), and D^{-1} would scale it past.
