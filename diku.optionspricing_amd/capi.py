@@ -24,8 +24,10 @@ FLAG_NO_GRAPH, FLAG_NO_ROWS, FLAG_CONCURRENT, FLAG_FUSED = 1, 2, 4, 8
 # The exported C symbols, in include/hwtree.h order.
 EXPORTS = (
     "hwt_config_init", "hwt_price_f64", "hwt_plan_create", "hwt_plan_execute",
-    "hwt_plan_sync", "hwt_plan_destroy", "hwt_alphas_f64", "hwt_autotune", "hwt_max_width",
-    "hwt_strerror", "hwt_last_error", "hwt_abi_version",
+    "hwt_plan_sync", "hwt_plan_destroy", "hwt_price_f32", "hwt_price_constants_f64", "hwt_price_constants_f32",
+    "hwt_batch_create", "hwt_batch_create_f32", "hwt_batch_price", "hwt_batch_device_memory", "hwt_batch_destroy",
+    "hwt_alphas_f64", "hwt_autotune", "hwt_max_width", "hwt_strerror",
+    "hwt_last_error", "hwt_abi_version",
 )
 
 
@@ -46,10 +48,28 @@ class YieldC(C.Structure):
     _fields_ = [("n", C.c_int32), ("rate", C.c_void_p), ("day", C.c_void_p)]
 
 
+# hwt_options_f32 / hwt_yield_f32: the same layouts with float arrays
+OptionsF32C, YieldF32C = OptionsC, YieldC
+
+
+class ConstantsC(C.Structure):
+    """hwt_option_constants_f64 (trinom::OptionConstants)."""
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("dr", C.c_double), ("X", C.c_double),
+                ("M", C.c_double), ("jmax", C.c_int32), ("n", C.c_int32), ("width", C.c_int32),
+                ("term_unit", C.c_uint16), ("type", C.c_int8)]
+
+
+class ConstantsF32C(C.Structure):
+    """hwt_option_constants_f32 (trinom::OptionConstants with real = float)."""
+    _fields_ = [("t", C.c_float), ("dt", C.c_float), ("dr", C.c_float), ("X", C.c_float),
+                ("M", C.c_float), ("jmax", C.c_int32), ("n", C.c_int32), ("width", C.c_int32),
+                ("term_unit", C.c_uint16), ("type", C.c_int8)]
+
+
 class ConfigC(C.Structure):
     _fields_ = [("form", C.c_int32), ("precision", C.c_int32), ("block_size", C.c_int32),
                 ("sort", C.c_int32), ("device", C.c_int32), ("flags", C.c_int32),
-                ("wide_per_cu", C.c_int32), ("narrow_per_cu", C.c_int32)]
+                ("wide_per_cu", C.c_int32), ("narrow_per_cu", C.c_int32), ("lane_max_width", C.c_int32)]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -69,7 +89,7 @@ class StatsC(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("wide_ms", C.c_double),
                 ("narrow_ms", C.c_double), ("device_memory", C.c_int64), ("node_updates", C.c_int64),
                 ("wide_node_updates", C.c_int64), ("seq_node_updates", C.c_int64), ("tasks", C.c_int32),
-                ("wide_tasks", C.c_int32), ("unsupported", C.c_int32), ("reserved", C.c_int32)]
+                ("wide_tasks", C.c_int32), ("unsupported", C.c_int32), ("fp64_repriced", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -93,6 +113,19 @@ def lib():
         L.hwt_plan_sync.argtypes = [P, C.POINTER(StatsC)]
         L.hwt_plan_destroy.argtypes = [P]
         L.hwt_plan_destroy.restype = None
+        L.hwt_price_f32.argtypes = [C.POINTER(OptionsF32C), C.POINTER(YieldF32C), P, C.POINTER(ConfigC),
+                                    C.POINTER(StatsC)]
+        L.hwt_price_constants_f64.argtypes = [P, C.c_int32, C.POINTER(YieldC), P, C.POINTER(ConfigC),
+                                              C.POINTER(StatsC)]
+        L.hwt_price_constants_f32.argtypes = [P, C.c_int32, C.POINTER(YieldF32C), P, C.POINTER(ConfigC),
+                                              C.POINTER(StatsC)]
+        L.hwt_batch_create.argtypes = [C.POINTER(P), C.POINTER(OptionsC), C.POINTER(YieldC)]
+        L.hwt_batch_create_f32.argtypes = [C.POINTER(P), C.POINTER(OptionsF32C), C.POINTER(YieldF32C)]
+        L.hwt_batch_price.argtypes = [P, P, C.POINTER(ConfigC), C.POINTER(StatsC)]
+        L.hwt_batch_device_memory.argtypes = [P]
+        L.hwt_batch_device_memory.restype = C.c_int64
+        L.hwt_batch_destroy.argtypes = [P]
+        L.hwt_batch_destroy.restype = None
         L.hwt_alphas_f64.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, P]
         L.hwt_autotune.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, C.POINTER(ConfigC),
                                    C.POINTER(TuneReportC)]
@@ -145,6 +178,94 @@ def price_status(opts: Options, y: Yield, cfg: ConfigC | None = None):
     rc = lib().hwt_price_f64(C.byref(_host_opts(opts)), C.byref(_host_yield(y)), out.ctypes.data,
                              C.byref(cfg) if cfg is not None else None, None)
     return rc, out
+
+
+def price_f32(opts: Options, y: Yield, cfg: ConfigC | None = None, stats: bool = False):
+    """hwt_price_f32: the batch's real fields as float32 host arrays (a
+    default-built reference caller's data), float32 prices back."""
+    f = {k: np.ascontiguousarray(getattr(opts, k), dtype=np.float32)
+         for k in ("strike", "maturity", "length", "a", "sigma")}
+    rates = np.ascontiguousarray(y.rates, dtype=np.float32)
+    o = OptionsF32C(opts.N, f["strike"].ctypes.data, f["maturity"].ctypes.data, f["length"].ctypes.data,
+                    opts.term_unit.ctypes.data, opts.term_steps.ctypes.data, f["a"].ctypes.data,
+                    f["sigma"].ctypes.data, opts.type.ctypes.data)
+    yc = YieldF32C(y.N, rates.ctypes.data, y.days.ctypes.data)
+    out = np.empty(opts.N, dtype=np.float32)
+    st = StatsC()
+    _check(lib().hwt_price_f32(C.byref(o), C.byref(yc), out.ctypes.data, C.byref(cfg) if cfg is not None else None,
+                               C.byref(st)))
+    return (out, st.as_dict()) if stats else out
+
+
+def constants_of(opts: Options) -> np.ndarray:
+    """Per-option trinom::OptionConstants (common/OptionConstants.hpp:29-53),
+    computed on the host in FP64 as an array of ConstantsC."""
+    arr = (ConstantsC * opts.N)()
+    for i in range(opts.N):
+        tuy = int(np.ceil(365.0 / float(opts.term_unit[i])))
+        steps = int(opts.term_steps[i])
+        n = int(steps * tuy * float(opts.maturity[i]))
+        dt = tuy / float(steps)
+        a, sg = float(opts.a[i]), float(opts.sigma[i])
+        V = sg * sg * (1.0 - np.exp(-2.0 * a * dt)) / (2.0 * a)
+        M = float(np.exp(-a * dt) - 1.0)
+        jmax = int(-0.184 / M) + 1
+        arr[i] = ConstantsC(float(opts.length[i]), dt, float(np.sqrt(3.0 * V)), float(opts.strike[i]), M, jmax, n,
+                            2 * jmax + 1, int(opts.term_unit[i]), int(opts.type[i]))
+    return arr
+
+
+def price_constants(consts, y: Yield, cfg: ConfigC | None = None, stats: bool = False):
+    """hwt_price_constants_f64 over an array of ConstantsC."""
+    n = len(consts)
+    out = np.empty(n, dtype=np.float64)
+    st = StatsC()
+    _check(lib().hwt_price_constants_f64(C.cast(consts, C.c_void_p), n, C.byref(_host_yield(y)), out.ctypes.data,
+                                         C.byref(cfg) if cfg is not None else None, C.byref(st)))
+    return (out, st.as_dict()) if stats else out
+
+
+def price_constants_f32(consts, y: Yield, cfg: ConfigC | None = None):
+    """hwt_price_constants_f32 over an array of ConstantsF32C (float prices)."""
+    n = len(consts)
+    out = np.empty(n, dtype=np.float32)
+    rates = np.ascontiguousarray(y.rates, dtype=np.float32)
+    yc = YieldF32C(y.N, rates.ctypes.data, y.days.ctypes.data)
+    _check(lib().hwt_price_constants_f32(C.cast(consts, C.c_void_p), n, C.byref(yc), out.ctypes.data,
+                                         C.byref(cfg) if cfg is not None else None, None))
+    return out
+
+
+class Batch:
+    """hwt_batch: a batch uploaded once and priced any number of times."""
+
+    def __init__(self, opts: Options, y: Yield):
+        self.N = opts.N
+        self._keep = (opts, y)
+        self._b = C.c_void_p()
+        _check(lib().hwt_batch_create(C.byref(self._b), C.byref(_host_opts(opts)), C.byref(_host_yield(y))))
+
+    def price(self, cfg: ConfigC | None = None, stats: bool = False):
+        out = np.empty(self.N, dtype=np.float64)
+        st = StatsC()
+        _check(lib().hwt_batch_price(self._b, out.ctypes.data, C.byref(cfg) if cfg is not None else None,
+                                     C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    @property
+    def device_memory(self) -> int:
+        return int(lib().hwt_batch_device_memory(self._b))
+
+    def close(self):
+        if self._b:
+            lib().hwt_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def alphas(opts: Options, y: Yield) -> list[np.ndarray]:

@@ -4,6 +4,8 @@
 //   hwt-seq     replaces Seq          (seq/SeqMain.cpp:12-81)
 //   hwt-option  replaces CudaOption   (cuda-option/CudaOptionMain.cu:10-119)
 //   hwt-multi   replaces CudaMulti    (cuda-multi/CudaMultiMain.cu:10-113)
+// each built with USE_DOUBLE (REAL=64) and without it (REAL=32, the "32"
+// binaries, which read and print floats with 9 decimals like the reference).
 //
 // Flags (common/Args.hpp:38-66): -o/--options FILE, -y/--yield FILE,
 // -s/--sort {-,w,W,h,H}..., -v/--version N..., -b/--block N..., -r/--runs N,
@@ -128,6 +130,11 @@ int main(int argc, char **argv)
         const trinom::Options options(args.options);
         const trinom::Yield yield(args.yield);
         hwtree::default_device() = args.device;
+#if HWT_CLI_MODE == 1
+        // this driver times the one-option-per-lane kernel, as CudaOption
+        // times the reference's one-option-per-thread kernel
+        hwtree::option_runner_form() = HWT_FORM_LANE;
+#endif
         for (int version : args.versions)
             for (int block : args.blocks)
                 for (trinom::SortType sort : args.sorts) {

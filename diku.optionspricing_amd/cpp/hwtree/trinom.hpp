@@ -5,9 +5,12 @@
 // trinom::Options (common/Options.hpp:48-125), trinom::Yield
 // (common/Yield.hpp:14-40), OptionType/SortType (common/Options.hpp:15-46)
 // and Arrays (common/Arrays.hpp:12-47), so existing callers compile unchanged.
-// `real` is double: this pricer computes in FP64.
+// `real` is float unless USE_DOUBLE is defined, as in the reference
+// (common/Real.hpp:13-21); the pricer computes in FP64 (or FP32 tree states
+// with FP64 sums) whichever width the caller holds its data in.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <fstream>
 #include <iostream>
@@ -19,7 +22,11 @@
 
 namespace trinom {
 
+#ifdef USE_DOUBLE
 typedef double real;
+#else
+typedef float real;
+#endif
 
 enum class SortType : char { WIDTH_DESC = 'W', WIDTH_ASC = 'w', HEIGHT_DESC = 'H', HEIGHT_ASC = 'h', NONE = '-' };
 
@@ -156,6 +163,49 @@ struct Yield {
         Arrays::read_array(in, Prices);
         Arrays::read_array(in, TimeSteps);
         N = static_cast<int>(Prices.size());
+    }
+};
+
+// An option's tree constants (common/OptionConstants.hpp:14-53), derived as
+// the reference derives them in either width of `real`, so a caller sees the
+// same height, jmax and width: `real` arithmetic, with the C library's double
+// exp, sqrt and ceil (what the reference's unqualified calls resolve to)
+// rounded to `real` where the fields store them.
+struct OptionConstants {
+    real t;
+    real dt;  // [years]
+    real dr;
+    real X;
+    real M;
+    int32_t jmax;
+    int32_t n;
+    int32_t width;
+    uint16_t termUnit;
+    OptionType type;
+
+    OptionConstants() {}
+
+    OptionConstants(const Options &options, const int ind)
+    {
+        termUnit = options.TermUnits.at(ind);
+        const real T = options.Maturities.at(ind);
+        const int termUnitsInYearCount =
+            static_cast<int>(std::ceil(static_cast<double>(static_cast<real>(365.0) / termUnit)));
+        const uint16_t termStepCount = options.TermStepCounts.at(ind);
+        n = static_cast<int32_t>(termStepCount * termUnitsInYearCount * T);
+        t = options.Lengths.at(ind);
+        dt = termUnitsInYearCount / static_cast<real>(termStepCount);
+        type = options.Types.at(ind);
+        const real a = options.ReversionRates.at(ind);
+        X = options.StrikePrices.at(ind);
+        const real sigma = options.Volatilities.at(ind);
+        const double V = sigma * sigma *
+                         (static_cast<real>(1) - std::exp(static_cast<double>(static_cast<real>(-2) * a * dt))) /
+                         (static_cast<real>(2) * a);
+        dr = static_cast<real>(std::sqrt(static_cast<real>(3) * V));
+        M = static_cast<real>(std::exp(static_cast<double>(-a * dt)) - static_cast<real>(1));
+        jmax = static_cast<int32_t>(static_cast<real>(-0.184) / M) + 1;
+        width = 2 * jmax + 1;
     }
 };
 

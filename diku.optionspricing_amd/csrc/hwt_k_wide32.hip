@@ -1,0 +1,10 @@
+// hwt_k_wide32.hip -- the wide-tree kernel with FP32 tree states (mixed mode).
+#include "hwt_pricers.h"
+
+namespace hwt {
+
+template __global__ void k_price_wide<float>(const OptConst *__restrict__, const double *__restrict__, const int32_t *__restrict__,
+                                           int32_t *__restrict__, double *__restrict__, double *__restrict__,
+                                           int32_t *__restrict__);
+
+} // namespace hwt

@@ -237,4 +237,36 @@ HWT_HD int32_t option_constants(double strike, double T, double len, uint16_t te
     return ST_OK;
 }
 
+// Constants given directly (the reference's OptionConstants, as
+// seq::computeSingleOption receives them, seq/Seq.hpp:24): the exercise step
+// as seq/Seq.hpp:125 derives it, plus the validation the other path does.
+// With f32, t and dt are floats widened to double and the step quotient is
+// taken in float, as a real = float caller takes it: the double quotient of
+// two floats rounds to the float quotient exactly (53 >= 2 * 24 + 2 bits).
+HWT_HD int32_t constants_given(double t, double dt, double dr, double X, double M, int32_t jmax, int32_t n,
+                               int32_t width, uint16_t termUnit, int8_t type, int32_t maxWidth, bool f32,
+                               OptConst &c)
+{
+    c.X = X;
+    c.type = type;
+    c.unit = 0;
+    c.Pe = c.Pn = c.rho = c.rhop = 0.0;
+    c.work = 0;
+    c.n = c.jmax = c.w = 0;
+    c.e = -1;
+    if ((type != 0 && type != 1) || termUnit == 0 || !(dt > 0.0) || !(dr >= 0.0) || !(M > -1.0 && M < 0.0) ||
+        jmax < 1 || width != 2 * jmax + 1 || n < 0 || !(X == X) || !(dr < 1e300) || !(t == t))
+        return ST_INVALID;
+    c.n = n;
+    c.dt = dt;
+    c.dr = dr;
+    c.M = M;
+    c.jmax = jmax;
+    c.w = width;
+    const double qe = f32 ? (double)(float)(t / dt) : t / dt;
+    c.e = (qe > -1.0 && qe < (double)c.n) ? (int32_t)qe : -1;
+    if (c.w > maxWidth) return ST_UNSUPPORTED;
+    return ST_OK;
+}
+
 } // namespace hwt

@@ -1,0 +1,95 @@
+// hwt_pipeline.h -- what the host side of the pricer and the pricing-kernel
+// translation units share: the device views of the batch, the task classes
+// and bucket layout, the counter slots, and the pricing kernels' signatures.
+// The kernels are defined in hwt_pricers.h and instantiated one per
+// translation unit (hwt_k_*.hip), so the library's slowest code compiles in
+// parallel.
+#pragma once
+
+#include "../../include/hwtree.h"
+#include "hwt_rows.h"
+#include "hwt_wave.h"
+
+namespace hwt {
+
+struct DevOpts {
+    int32_t n;  // options, or the capacity bound when `count` is set
+    const double *strike, *maturity, *length;
+    const uint16_t *term_unit, *term_steps;
+    const double *a, *sigma;
+    const int8_t *type;
+    const int32_t *count;  // device-side option count (<= n), or null
+    // per-option tree constants given by the caller instead of the fields
+    // above (which are then null), or null
+    const hwt_option_constants_f64 *consts;
+    // with consts: the constants are a float caller's, widened; the exercise
+    // step is then (int)(t / dt) in float arithmetic
+    int32_t consts_f32;
+};
+
+struct DevYield {
+    int32_t n;
+    const double *rate;
+    const int32_t *day;
+};
+
+// Task classes, in task-list order:
+//   wide  (w > 64*KSMALL)           k_price_wide, one option per wave, LPT queue
+//   mid   (ROW_MAX_WIDTH < w <= 64*KSMALL)  k_price_narrow, one option per wave
+//   row   (w <= ROW_MAX_WIDTH)      k_price_narrow, four options per wave
+// Bucket ranges: wide [B_WIDE, 1024) and mid [B_MID, B_WIDE), log-spaced by
+// the sort key (work unless a width/height order is asked for); row
+// [0, B_MID), by (nodes per lane, height) so the four rows of a wave share
+// their step count.  The scan walks each class from its largest key down.
+constexpr int NBUCKET = 1024;
+constexpr int B_MID = 640, B_WIDE = 768;
+constexpr int KSMALL = 2;  // trees up to 64*KSMALL nodes wide go to the narrow-tree kernel
+#ifndef WIDE_WINDOW_STEP
+#define WIDE_WINDOW_STEP 4  // nodes per lane a wide tree's window grows or shrinks by
+#endif
+enum Tally {
+    T_QUEUE = 0,    // next task of the wide-tree kernel (tasks [0, T_NLARGE))
+    T_QUEUE_S = 1,  // (unused: the narrow-tree kernel strides statically)
+    T_NTASK = 2,
+    T_NLARGE = 3,
+    T_INVALID = 4,
+    T_UNSUP = 5,
+    T_WORK = 6,     // int64
+    T_SEQWORK = 8,  // int64
+    T_WORKW = 10,   // int64, wide trees only
+    T_NMID = 12,    // end of the mid-class tasks
+    T_NPAD = 13,    // idle slots padding the row class
+    T_NCOUNT = 14,  // end of the execution counts (slots 16-23 are k_setup's accumulators)
+    T_QUEUE_R = 15, // next chunk of row groups (fused kernel)
+    // accumulated by k_setup's blocks, then moved into the slots above and
+    // cleared by k_scan
+    A_WORK = 16,    // int64
+    A_SEQWORK = 18, // int64
+    A_WORKW = 20,   // int64
+    A_INVALID = 22,
+    A_UNSUP = 23,
+    T_NREDO = 24,   // mixed mode: options handed to the FP64 re-pricing (host-visible)
+    T_QUEUE_X = 25, // next task of the FP64 re-pricing kernel
+    T_NCOUNT_ALL = 26,  // [0, T_NCOUNT_ALL) is copied to the host after an execution
+    T_ALLOC = 32
+};
+
+// Pricing kernels (hwt_pricers.h).  All are persistent: one single-wave
+// workgroup per resident slot, tasks from the ordered task list.
+template <class T>
+__global__ void k_price_wide(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                             const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                             double *__restrict__ out, double *__restrict__ scratch, int32_t *__restrict__ redo);
+template <class T>
+__global__ void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                               const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                               double *__restrict__ out, int32_t *__restrict__ redo);
+template <class T>
+__global__ void k_price_fused(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                              const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                              double *__restrict__ out, double *__restrict__ scratch, int32_t *__restrict__ redo);
+__global__ void k_price_redo(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                             const int32_t *__restrict__ redo, int32_t *__restrict__ tally,
+                             double *__restrict__ out, double *__restrict__ scratch);
+
+} // namespace hwt

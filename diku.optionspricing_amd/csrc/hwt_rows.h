@@ -220,15 +220,23 @@ __device__ __forceinline__ double price_rows(const RowOpt &r, int l, int e_row, 
     __syncthreads();
     double yv[4], bv[4];
     typename SumOf<T>::type a1, a2, a3;
+    bool big = false;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         yv[s] = ybuf[4 * l + s];
         bv[s] = bbuf[4 * l + s];
         a1.add(yv[s]);
         a2.add(yv[s] * bv[s]);
+        big |= !(fabs(bv[s]) <= F32_SAFE_STATE);
     }
     __syncthreads();  // the buffer is reused by the wave's next group
     const double s1 = a1.row_total(), s2 = a2.row_total();
+    if constexpr (sizeof(T) == sizeof(float)) {
+        // FP32 state of this row's option out of range: NaN asks for the
+        // FP64 re-pricing (the row's 16 bits of the ballot)
+        const int row = __lane_id() / ROW;
+        if (((__ballot(big) >> (ROW * row)) & 0xFFFFull) != 0 || !isfinite(s1) || !isfinite(s2)) return NAN;
+    }
     const double scale = 100.0 * s1 * r.Pn / (s2 * r.Pe);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {

@@ -189,6 +189,16 @@ __device__ __forceinline__ float dpp_from_upper_lane(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 
+// Mixed mode's range guard.  The undiscounted backward (bond) state of a long,
+// weakly mean-reverting bond grows without bound at the low-rate edge of the
+// tree: 1e40 to 1e80 in FP64 for 100-year bonds at a ~ 0.01, past FP32's
+// 3.4e38.  An overflowed node turns its light cone into inf/NaN, so the FP32
+// pricers report an option as NaN when a duality sum is not finite or a bond
+// value at an exercise node exceeds this bound, and the option is re-priced by
+// the FP64 kernel.  Below the bound a forward value lost to FP32 underflow
+// (< 1.2e-38) changes S2 by less than 1e-9 per node.
+constexpr double F32_SAFE_STATE = 0x1p96;
+
 // Largest finite value of the propagation type: a payoff that overflows is
 // clamped to it, as the reference does in either precision
 // (common/Domain.hpp:163-168).

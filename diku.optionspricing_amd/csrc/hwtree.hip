@@ -25,8 +25,7 @@
 // So each option costs n stencil steps (e forward + n-e backward) with no
 // per-step reductions, against 2n steps plus n reductions in seq/Seq.hpp.
 #include "hwt_lane.h"
-#include "hwt_rows.h"
-#include "hwt_wave.h"
+#include "hwt_pipeline.h"
 #include "../../include/hwtree.h"
 #include "hwt_internal.h"
 
@@ -50,59 +49,6 @@
 using namespace hwt;
 
 namespace {
-
-struct DevOpts {
-    int32_t n;  // options, or the capacity bound when `count` is set
-    const double *strike, *maturity, *length;
-    const uint16_t *term_unit, *term_steps;
-    const double *a, *sigma;
-    const int8_t *type;
-    const int32_t *count;  // device-side option count (<= n), or null
-};
-
-struct DevYield {
-    int32_t n;
-    const double *rate;
-    const int32_t *day;
-};
-
-// Task classes, in task-list order:
-//   wide  (w > 64*KSMALL)           k_price_wide, one option per wave, LPT queue
-//   mid   (ROW_MAX_WIDTH < w <= 64*KSMALL)  k_price_narrow, one option per wave
-//   row   (w <= ROW_MAX_WIDTH)      k_price_narrow, four options per wave
-// Bucket ranges: wide [B_WIDE, 1024) and mid [B_MID, B_WIDE), log-spaced by
-// the sort key (work unless a width/height order is asked for); row
-// [0, B_MID), by (nodes per lane, height) so the four rows of a wave share
-// their step count.  The scan walks each class from its largest key down.
-constexpr int NBUCKET = 1024;
-constexpr int B_MID = 640, B_WIDE = 768;
-constexpr int KSMALL = 2;  // trees up to 64*KSMALL nodes wide go to the narrow-tree kernel
-#ifndef WIDE_WINDOW_STEP
-#define WIDE_WINDOW_STEP 4  // nodes per lane a wide tree's window grows or shrinks by
-#endif
-enum Tally {
-    T_QUEUE = 0,    // next task of the wide-tree kernel (tasks [0, T_NLARGE))
-    T_QUEUE_S = 1,  // (unused: the narrow-tree kernel strides statically)
-    T_NTASK = 2,
-    T_NLARGE = 3,
-    T_INVALID = 4,
-    T_UNSUP = 5,
-    T_WORK = 6,     // int64
-    T_SEQWORK = 8,  // int64
-    T_WORKW = 10,   // int64, wide trees only
-    T_NMID = 12,    // end of the mid-class tasks
-    T_NPAD = 13,    // idle slots padding the row class
-    T_NCOUNT = 14,  // [0, T_NCOUNT) is what the host reads after an execution
-    T_QUEUE_R = 15, // next chunk of row groups (fused kernel)
-    // accumulated by k_setup's blocks, then moved into the slots above and
-    // cleared by k_scan
-    A_WORK = 16,    // int64
-    A_SEQWORK = 18, // int64
-    A_WORKW = 20,   // int64
-    A_INVALID = 22,
-    A_UNSUP = 23,
-    T_ALLOC = 32
-};
 
 __device__ __forceinline__ int log_bucket(int64_t key, int per_octave, int nb)
 {
@@ -223,6 +169,8 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
         t64[T_WORKW / 2] = t64[A_WORKW / 2];
         tally[T_INVALID] = tally[A_INVALID];
         tally[T_UNSUP] = tally[A_UNSUP];
+        tally[T_NREDO] = 0;
+        tally[T_QUEUE_X] = 0;
         t64[A_WORK / 2] = 0;
         t64[A_SEQWORK / 2] = 0;
         t64[A_WORKW / 2] = 0;
@@ -251,7 +199,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     double strike = 0.0, mat = 0.0, len = 0.0, a = 0.0, sigma = 0.0;
     uint16_t tu = 0, ts = 0;
     int8_t type = 0;
-    if (mine) {
+    if (mine && !o.consts) {
         strike = o.strike[i];
         mat = o.maturity[i];
         len = o.length[i];
@@ -276,13 +224,24 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
             if (k + 1 < y.n && y.day[k] > y.day[k + 1]) unsorted = 1;
         }
     }
-    const bool sorted = staged && !__syncthreads_or(unsorted);
+    // the barrier runs unconditionally: it also orders the LDS clearing above
+    // before the atomics below, for curves too long to stage as well
+    const int any_unsorted = __syncthreads_or(unsorted);
+    const bool sorted = staged && !any_unsorted;
     auto yield = [&](double t, int tu) {
         return sorted ? yield_at_sorted(t, tu, yr, yd, y.n) : yield_at(t, tu, y.rate, y.day, y.n);
     };
     if (mine) {
         OptConst c;
-        const int32_t st = option_constants(strike, mat, len, tu, ts, a, sigma, type, maxWidth, c);
+        int32_t st;
+        if (o.consts) {
+            const hwt_option_constants_f64 k = o.consts[i];
+            tu = k.term_unit;
+            st = constants_given(k.t, k.dt, k.dr, k.X, k.M, k.jmax, k.n, k.width, k.term_unit, k.type, maxWidth,
+                                 o.consts_f32 != 0, c);
+        } else {
+            st = option_constants(strike, mat, len, tu, ts, a, sigma, type, maxWidth, c);
+        }
         c.status = st;
         int b = -1;
         if (st == ST_INVALID) {
@@ -376,282 +335,20 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_
     if (b >= 0) order[offs[b] + lbase[b] + r] = i;
 }
 
-// Payoff of the option on a bond worth B at the exercise step, clamped to the
-// largest finite value of the propagation type (common/Domain.hpp:163-168).
-template <class T>
-__device__ __forceinline__ double payoff(const OptConst &cv, double B)
-{
-    double v = (cv.type == 1) ? cv.X - B : B - cv.X;  // PUT : CALL
-    v = (v < 0.0) ? 0.0 : v;
-    return (v > RealMax<T>::value) ? RealMax<T>::value : v;
-}
-
-// Price from the two states at the exercise step: y (forward, Arrow-Debreu)
-// and b (backward, bond value per unit face), both untransformed:
-//   S1 = sum y, S2 = sum y b, scale = 100 S1 Pn / (S2 Pe),
-//   price = Pe / S1 * sum y payoff(scale b).
-// Sums are FP64 (compensated in mixed mode).
-
-template <class T, int KMAX>
-__device__ __forceinline__ double price_option(const OptConst &cv, const double *edge, int lane,
-                                               typename Lds<T>::type *lds, lds_f64 *etab, T *fscratch)
-{
-    const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
-    const int n = uni(cv.n), e = uni(cv.e);
-    load_edge_table(edge, lane, etab);
-    int flo, fhi, blo, bhi;
-    const bool unit = uni((int)cv.unit) != 0;
-    // wide trees change window four nodes per lane at a time (a 60k batch of
-    // wide and narrow trees: 283 us with single steps, 252 us with 2, 243 us
-    // with 4); the mid class (up to 128 nodes wide) keeps single steps
-    constexpr int G = KMAX > KSMALL ? WIDE_WINDOW_STEP : 1;
-    forward_pass<T, KMAX, G>(t, e, lane, lds, etab, fscratch, flo, fhi, unit);
-    backward_pass<T, KMAX, G>(t, n, e, lane, lds, etab, blo, bhi, unit);
-
-    const int ep = min(e, t.jmax);
-    const int klo = t.jmax - ep, khi = t.jmax + ep;
-    auto fy = [&](int k) { return (k >= flo && k < fhi) ? (double)fscratch[k] : 0.0; };
-    auto fb = [&](int k) { return (k >= blo && k < bhi) ? (double)lds[k] : 0.0; };
-    auto yk = [&](int k) {
-        double v = fy(k);
-        if (k == 1) v -= t.rho * fy(0);
-        if (k == t.w - 2) v -= t.rhop * fy(t.w - 1);
-        return v;
-    };
-    auto bk = [&](int k) {
-        double v = fb(k);
-        if (k == 0) v += t.rho * fb(1);
-        if (k == t.w - 1) v += t.rhop * fb(t.w - 2);
-        return v;
-    };
-    typename SumOf<T>::type a1, a2, a3;
-    for (int k = klo + lane; k <= khi; k += WAVE) {
-        const double yv = yk(k);
-        a1.add(yv);
-        a2.add(yv * bk(k));
-    }
-    const double s1 = a1.wave_total(), s2 = a2.wave_total();
-    const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
-    for (int k = klo + lane; k <= khi; k += WAVE) a3.add(yk(k) * payoff<T>(cv, scale * bk(k)));
-    const double s3 = a3.wave_total();
-    __syncthreads();
-    return cv.Pe / s1 * s3;
-}
-
-// Narrow trees (width <= 64*K): the whole tree in one full window, both passes
-// in registers, and the duality sums formed straight from the registers.
-template <class T, int K>
-__device__ __forceinline__ double price_narrow(const OptConst &cv, const double *edge, int lane, lds_f64 *etab)
-{
-    const TreeShape t = uni(TreeShape{cv.w, cv.jmax, cv.M, cv.dr, cv.dt, cv.rho, cv.rhop});
-    const int n = uni(cv.n), e = uni(cv.e);
-    load_edge_table(edge, lane, etab);
-    T cl[K], cm[K], cr[K], zf[K], zb[K];
-    window_coefs<T, K, true>(t, 0, lane, etab, cl, cm, cr);
-    init_window<T, K, true>(zf, t, 0, lane);
-    stencil_steps<T, K>(zf, cl, cm, cr, e);
-    window_coefs<T, K, false>(t, 0, lane, etab, cl, cm, cr);
-    init_window<T, K, false>(zb, t, 0, lane);
-    stencil_steps<T, K>(zb, cl, cm, cr, n - e);
-
-    // undo the edge similarity: y = T^{-1} zf, b = T^T zb
-    const double fl = dpp_from_lower_lane(zf[K - 1]), fr = dpp_from_upper_lane(zf[0]);
-    const double bl = dpp_from_lower_lane(zb[K - 1]), br = dpp_from_upper_lane(zb[0]);
-    double yv[K], bv[K];
-    typename SumOf<T>::type a1, a2, a3;
-#pragma unroll
-    for (int s = 0; s < K; ++s) {
-        const int k = lane * K + s;
-        const double fL = s > 0 ? (double)zf[s - 1] : fl, fR = s + 1 < K ? (double)zf[s + 1] : fr;
-        const double bL = s > 0 ? (double)zb[s - 1] : bl, bR = s + 1 < K ? (double)zb[s + 1] : br;
-        double y = zf[s], b = zb[s];
-        if (k == 1) y -= t.rho * fL;
-        if (k == t.w - 2) y -= t.rhop * fR;
-        if (k == 0) b += t.rho * bR;
-        if (k == t.w - 1) b += t.rhop * bL;
-        if (k >= t.w) y = b = 0.0;
-        yv[s] = y;
-        bv[s] = b;
-        a1.add(y);
-        a2.add(y * b);
-    }
-    const double s1 = a1.wave_total(), s2 = a2.wave_total();
-    const double scale = 100.0 * s1 * cv.Pn / (s2 * cv.Pe);
-#pragma unroll
-    for (int s = 0; s < K; ++s) a3.add(yv[s] * payoff<T>(cv, scale * bv[s]));
-    const double s3 = a3.wave_total();
-    return cv.Pe / s1 * s3;
-}
-
-// Persistent pricing kernels: one single-wave workgroup per resident slot;
-// the waves work through tasks [first, last) of the task list.
-//   DYNAMIC (wide trees): each wave takes task first + blockIdx.x, then claims
-//     further tasks from the queue counter, so the long trees at the head of
-//     the list balance themselves.  Lane 0 stores the finished price and
-//     claims the next task in one divergent region; every lane then reads the
-//     claim back through a convergent broadcast, so all 64 lanes always leave
-//     the loop together.
-//   static (narrow trees): wave b takes tasks first + b, first + b + grid, ...
-//     Narrow tasks are many and short, and one counter bumped once per task
-//     by every wave serialises in L2; the list is in decreasing work order, so
-//     a fixed stride spreads the work evenly.
-template <class T, int KMAX, bool NARROW, bool DYNAMIC>
-__device__ __forceinline__ void price_tasks(const OptConst *__restrict__ cst, const double *__restrict__ edge,
-                                            const int32_t *__restrict__ order, int32_t *__restrict__ tally,
-                                            int queue, int first, int last, double *__restrict__ out,
-                                            double *__restrict__ scratch)
-{
-    __shared__ T lds[NARROW ? 1 : WAVE * KMAX];
-    __shared__ double etab[EDGE_TAB];
-    const int lane = threadIdx.x;
-    T *fscratch = reinterpret_cast<T *>(scratch + (size_t)blockIdx.x * (WAVE * KMAX));
-    const int stride = uni((int)gridDim.x);
-    int t = uni((int)blockIdx.x) + first;
-    while (t < last) {
-        const int o = uni(order[t]);
-        const OptConst c = cst[o];
-        const double *eo = edge + (size_t)o * EDGE_TAB;
-        double p;
-        if (NARROW) {
-            const int K = (uni(c.w) + WAVE - 1) / WAVE;
-            p = (K == 1) ? price_narrow<T, 1>(c, eo, lane, (lds_f64 *)etab)
-                         : price_narrow<T, (KMAX >= 2 ? 2 : 1)>(c, eo, lane, (lds_f64 *)etab);
-        } else {
-            p = price_option<T, KMAX>(c, eo, lane, (typename Lds<T>::type *)lds, (lds_f64 *)etab, fscratch);
-        }
-        if (DYNAMIC) {
-            int claim = 0;
-            if (lane == 0) {
-                out[o] = p;
-                claim = atomicAdd(&tally[queue], 1);
-            }
-            t = uni(__shfl(claim, 0, WAVE)) + first + stride;
-        } else {
-            if (lane == 0) out[o] = p;
-            t += stride;
-        }
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void k_price_wide(
-    const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
-    int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
-{
-    const int nlarge = uni(tally[T_NLARGE]);
-    price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
-}
-
-// Row-class tasks [first, last) in groups of four consecutive tasks, one
-// group per wave at a time (static stride); the last group may hold fewer
-// options, and its idle rows repeat the group's first option without
-// writing a price.
-//   DYNAMIC: groups are claimed ROW_CHUNK at a time from tally[T_QUEUE_R]
-//   (the fused kernel, whose wavefronts reach the row class at different
-//   times); otherwise wave b takes groups b, b + grid, ...
-constexpr int ROW_CHUNK = 4;
-template <class T, bool DYNAMIC>
-__device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cst, const double *__restrict__ edge,
-                                                 const int32_t *__restrict__ order, int first, int last,
-                                                 double *__restrict__ out, int32_t *__restrict__ tally)
-{
-    __shared__ double rowbuf[2 * ROWS * ROW_MAX_WIDTH];
-    const int lane = threadIdx.x, row = lane / ROW, l = lane % ROW;
-    lds_f64 *ybuf = (lds_f64 *)rowbuf + row * ROW_MAX_WIDTH;
-    lds_f64 *bbuf = (lds_f64 *)rowbuf + (ROWS + row) * ROW_MAX_WIDTH;
-    const int ngroups = (last - first + ROWS - 1) / ROWS;
-    int g, gend;
-    auto claim = [&]() {
-        int c = 0;
-        if (lane == 0) c = atomicAdd(&tally[T_QUEUE_R], 1);
-        c = uni(__shfl(c, 0, WAVE));
-        g = c * ROW_CHUNK;
-        gend = min(g + ROW_CHUNK, ngroups);
-    };
-    if (DYNAMIC) claim();
-    else g = uni((int)blockIdx.x);
-    while (g < ngroups) {
-        const int base = first + g * ROWS;
-        const int mine = base + row < last ? order[base + row] : -1;
-        const bool active = mine >= 0;
-        const int o = active ? mine : order[base];  // a group's first slot is never idle
-        const OptConst c = cst[o];
-        RowOpt r;
-        r.t = TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
-        r.n = c.n;
-        r.e = c.e;
-        r.type = c.type;
-        r.X = c.X;
-        r.Pe = c.Pe;
-        r.Pn = c.Pn;
-        r.edge = edge + (size_t)o * EDGE_TAB;
-        const int e_row = active ? c.e : 0, n_row = active ? c.n : 0;
-        int ev[8];
-        int K = 1;
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-            ev[q] = __builtin_amdgcn_readlane(e_row, q * ROW);
-            ev[ROWS + q] = __builtin_amdgcn_readlane(n_row, q * ROW);
-            K = max(K, (__builtin_amdgcn_readlane(c.w, q * ROW) + ROW - 1) / ROW);
-        }
-        double p;
-        switch (K) {
-        case 1: p = price_rows<T, 1>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
-        case 2: p = price_rows<T, 2>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
-        case 3: p = price_rows<T, 3>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
-        default: p = price_rows<T, 4>(r, l, e_row, n_row, ev, ybuf, bbuf); break;
-        }
-        if (active && l == 0) out[o] = p;
-        if (DYNAMIC) {
-            if (++g >= gend) claim();
-        } else {
-            g += uni((int)gridDim.x);
-        }
-    }
-}
-
-// Four waves per SIMD (at most 128 VGPRs): on a 1M batch of mostly tall
-// narrow trees the kernel runs 5% faster than at the three its 149 VGPRs
-// would otherwise allow.
-template <class T>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_price_narrow(const OptConst *__restrict__ cst, const double *__restrict__ edge,
-                                                       const int32_t *__restrict__ order, int32_t *__restrict__ tally,
-                                                       double *__restrict__ out)
-{
-    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
-    price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr);
-    price_row_groups<T, false>(cst, edge, order, nmid, ntask, out, tally);
-}
-
-// Every class in one persistent kernel: the wide trees from the LPT queue,
-// then the mid class and the row groups from their own queues, so waves that
-// run out of wide trees early price the narrow ones while the last wide
-// trees finish.
-template <class T>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void k_price_fused(
-    const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
-    int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch)
-{
-    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
-    price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch);
-    price_tasks<T, KSMALL, true, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr);
-    price_row_groups<T, true>(cst, edge, order, nmid, ntask, out, tally);
-}
-
 // Form 1: one lane per option over every scheduled task (tasks ordered by
 // decreasing work, so the lanes of a wave carry similar trees).
 __global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__restrict__ edge,
                              const int32_t *__restrict__ order, const int32_t *__restrict__ tally,
-                             double *__restrict__ out, double *__restrict__ cols)
+                             double *__restrict__ out, double *__restrict__ cols, int32_t colw)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= tally[T_NTASK]) return;
     const int o = order[gid];
     if (o < 0) return;  // idle pad slot
     const OptConst c = cst[o];
-    double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)LANE_MAX_WIDTH * WAVE) + (gid % WAVE);
+    double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)colw * WAVE) + (gid % WAVE);
     out[o] = lane_price(c, edge + (size_t)o * EDGE_TAB, LaneColumn{base},
-                        LaneColumn{base + (size_t)LANE_MAX_WIDTH * WAVE});
+                        LaneColumn{base + (size_t)colw * WAVE});
 }
 
 // ------------------------------------------------------------- alphas
@@ -761,8 +458,11 @@ struct hwt_plan {
     hipStream_t side = nullptr;       // narrow-tree kernel runs here, concurrently
     hipEvent_t fork = nullptr, join = nullptr;
     double *scratch_wide = nullptr;
+    int32_t *redo = nullptr;  // mixed mode: options re-priced in FP64
+    int grid_redo = 0;
     double *cols = nullptr;  // form 1 tree columns
     int lane_block = 256;
+    int lane_width = LANE_MAX_WIDTH;  // form 1: column length, the widest tree priced
     // device workspace, one allocation
     void *ws = nullptr;
     size_t ws_bytes = 0;
@@ -802,59 +502,69 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
         const bool rows = p->cfg.form != HWT_FORM_LANE && !(p->cfg.flags & HWT_FLAG_NO_ROWS);
         // the lane form prices tasks one per lane: no row groups, no pads
-        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, WAVE_MAX_WIDTH, unit_dmin, rows, p->cst, p->edge,
+        // form 1 prices trees up to its column length; wider ones are unsupported
+        const int max_width = p->cfg.form == HWT_FORM_LANE ? p->lane_width : WAVE_MAX_WIDTH;
+        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, max_width, unit_dmin, rows, p->cst, p->edge,
                                                p->bucket, p->hist, p->tally, out);
         k_scan<<<1, SETUP_BLOCK, 0, s>>>(p->hist, p->offs, p->cursor, p->order, p->tally, asc, rows);
         k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
             const int lb = p->lane_block;
-            k_price_lane<<<(n + lb - 1) / lb, lb, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->cols);
+            k_price_lane<<<(n + lb - 1) / lb, lb, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->cols,
+                                                          p->lane_width);
             if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
             HWT_HIP(hipGetLastError());
             return HWT_OK;
         }
+        const bool mixed = p->cfg.precision == HWT_PREC_MIXED;
         if (p->cfg.flags & HWT_FLAG_FUSED) {
             if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
-            if (p->cfg.precision == HWT_PREC_MIXED)
+            if (mixed)
                 k_price_fused<float><<<p->grid_fused, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
-                                                                    p->scratch_wide);
+                                                                    p->scratch_wide, p->redo);
             else
                 k_price_fused<double><<<p->grid_fused, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
-                                                                     p->scratch_wide);
+                                                                     p->scratch_wide, p->redo);
             if (time_kernel) {
                 HWT_HIP(hipEventRecord(p->evw1, s));
                 HWT_HIP(hipEventRecord(p->evn0, s));  // no separate narrow kernel: narrow_ms ~ 0
                 HWT_HIP(hipEventRecord(p->evn1, s));
-                HWT_HIP(hipEventRecord(p->evk1, s));
             }
-            HWT_HIP(hipGetLastError());
-            return HWT_OK;
+        } else {
+            // the narrow-tree kernel after the wide-tree one on `s`, or beside it
+            // on the side stream (HWT_FLAG_CONCURRENT)
+            const bool serial = (p->cfg.flags & HWT_FLAG_CONCURRENT) == 0;
+            hipStream_t ns = serial ? s : p->side;
+            if (!serial) {
+                HWT_HIP(hipEventRecord(p->fork, s));
+                HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+            }
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
+            if (mixed)
+                k_price_wide<float><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                  p->scratch_wide, p->redo);
+            else
+                k_price_wide<double><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                   p->scratch_wide, p->redo);
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evw1, s));
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, ns));
+            if (mixed)
+                k_price_narrow<float><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                       p->redo);
+            else
+                k_price_narrow<double><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out,
+                                                                        p->redo);
+            if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, ns));
+            if (!serial) {
+                HWT_HIP(hipEventRecord(p->join, p->side));
+                HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
+            }
         }
-        // the narrow-tree kernel after the wide-tree one on `s`, or beside it
-        // on the side stream (HWT_FLAG_CONCURRENT)
-        const bool serial = (p->cfg.flags & HWT_FLAG_CONCURRENT) == 0;
-        hipStream_t ns = serial ? s : p->side;
-        if (!serial) {
-            HWT_HIP(hipEventRecord(p->fork, s));
-            HWT_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
-        }
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evw0, s));
-        if (p->cfg.precision == HWT_PREC_MIXED)
-            k_price_wide<float><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
-        else
-            k_price_wide<double><<<p->grid_wide, WAVE, 0, s>>>(p->cst, p->edge, p->order, p->tally, out, p->scratch_wide);
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evw1, s));
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evn0, ns));
-        if (p->cfg.precision == HWT_PREC_MIXED)
-            k_price_narrow<float><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out);
-        else
-            k_price_narrow<double><<<p->grid_narrow, WAVE, 0, ns>>>(p->cst, p->edge, p->order, p->tally, out);
-        if (time_kernel) HWT_HIP(hipEventRecord(p->evn1, ns));
-        if (!serial) {
-            HWT_HIP(hipEventRecord(p->join, p->side));
-            HWT_HIP(hipStreamWaitEvent(s, p->join, 0));
-        }
+        // mixed mode: FP64 re-pricing of the options whose FP32 state left its
+        // range, after every FP32 kernel has handed its options over
+        if (mixed)
+            k_price_redo<<<p->grid_redo, WAVE, 0, s>>>(p->cst, p->edge, p->redo, p->tally, out, p->scratch_wide);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk1, s));
     }
     HWT_HIP(hipGetLastError());
@@ -956,6 +666,13 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
         per_cu_f < 1)
         per_cu_f = per_cu_w;
     p->grid_fused = std::min(per_cu_f, per_cu_w) * cus;
+    // mixed mode's FP64 re-pricing kernel shares the wide kernel's scratch
+    // slots as well
+    int per_cu_x = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_x, (const void *)k_price_redo, WAVE, 0) != hipSuccess ||
+        per_cu_x < 1)
+        per_cu_x = 1;
+    p->grid_redo = std::min(per_cu_x, per_cu_w) * cus;
     if (const char *v = getenv("HWT_VERBOSE"); v && *v && *v != '0')
         fprintf(stderr, "hwtree plan: device %d, %d CUs, wide kernel %d blocks (%d/CU), narrow kernel %d blocks (%d/CU)\n",
                 p->device, cus, p->grid_wide, per_cu_w, p->grid_narrow, per_cu_n);
@@ -974,14 +691,18 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
     const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));
-    const size_t o_scw = carve((size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
     const bool lane = p->cfg.form == HWT_FORM_LANE;
+    // the wide-tree kernels' parked forward states (form 2 only)
+    const size_t o_scw = carve(lane ? 0 : (size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
+    const size_t o_redo = carve(p->cfg.precision == HWT_PREC_MIXED ? cap * sizeof(int32_t) : 0);
     if (lane) {
         const int bs = p->cfg.block_size;
         p->lane_block = (bs >= WAVE && bs <= 1024 && bs % WAVE == 0) ? bs : 256;
+        const int lw = p->cfg.lane_max_width;
+        p->lane_width = (lw > 0 && lw < LANE_MAX_WIDTH) ? lw : LANE_MAX_WIDTH;
     }
     const size_t lane_waves = lane ? (cap + WAVE - 1) / WAVE : 0;
-    const size_t o_cols = carve(lane_waves * 2 * (size_t)LANE_MAX_WIDTH * WAVE * sizeof(double));
+    const size_t o_cols = carve(lane_waves * 2 * (size_t)p->lane_width * WAVE * sizeof(double));
     p->ws_bytes = off;
     if (hipMalloc(&p->ws, p->ws_bytes) != hipSuccess) return bail(HWT_ENOMEM, "hipMalloc workspace failed");
     char *base = (char *)p->ws;
@@ -994,6 +715,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     p->offs = (int32_t *)(base + o_offs);
     p->tally = (int32_t *)(base + o_tally);
     p->scratch_wide = (double *)(base + o_scw);
+    p->redo = p->cfg.precision == HWT_PREC_MIXED ? (int32_t *)(base + o_redo) : nullptr;
     p->cols = lane ? (double *)(base + o_cols) : nullptr;
     // counters start clear; from then on every execution leaves them clear
     if (hipMemsetAsync(p->hist, 0, NBUCKET * sizeof(int32_t), p->stream) != hipSuccess ||
@@ -1001,9 +723,9 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
         hipMemsetAsync(p->tally, 0, T_ALLOC * sizeof(int32_t), p->stream) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess)
         return bail(HWT_EHIP, "clearing the plan's counters failed");
-    if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT_ALL * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
-    std::memset(p->h_tally, 0, T_NCOUNT * sizeof(int32_t));  // a plan never executed syncs clean
+    std::memset(p->h_tally, 0, T_NCOUNT_ALL * sizeof(int32_t));  // a plan never executed syncs clean
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
         hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
         hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||
@@ -1035,16 +757,13 @@ void hwt_plan_destroy(hwt_plan *p)
     delete p;
 }
 
-int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_f64 *dyield, double *dout, void *stream)
+} // extern "C"
+
+// Enqueues one execution over device views (graph replay unless
+// HWT_FLAG_NO_GRAPH; the graph is captured again when the views or the
+// stream change).
+static int plan_execute_views(hwt_plan *p, const DevOpts &o, const DevYield &y, double *dout, hipStream_t s)
 {
-    g_last_error.clear();
-    if (!p || !valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
-        return fail(HWT_EINVAL, "hwt_plan_execute: null or negative-size argument");
-    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "hwt_plan_execute: more options than the plan capacity");
-    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "hwt_plan_execute: yield curve longer than the plan capacity");
-    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    const DevOpts o = dev_opts(*dopts);
-    const DevYield y{dyield->n, dyield->rate, dyield->day};
     p->last_stream = s;
     HWT_HIP(hipEventRecord(p->ev0, s));
     if (p->cfg.flags & HWT_FLAG_NO_GRAPH) {
@@ -1081,6 +800,19 @@ int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_
     return HWT_OK;
 }
 
+extern "C" {
+
+int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_f64 *dyield, double *dout, void *stream)
+{
+    g_last_error.clear();
+    if (!p || !valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
+        return fail(HWT_EINVAL, "hwt_plan_execute: null or negative-size argument");
+    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "hwt_plan_execute: more options than the plan capacity");
+    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "hwt_plan_execute: yield curve longer than the plan capacity");
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return plan_execute_views(p, dev_opts(*dopts), DevYield{dyield->n, dyield->rate, dyield->day}, dout, s);
+}
+
 int hwt_internal_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
 
 // In-stream variant for callers that only know the option count on the
@@ -1115,7 +847,7 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
     // the tallies are fetched here, once, rather than by a copy queued behind
     // every execution: back-to-back executions then run without a transfer
     // between them
-    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT_ALL * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HWT_HIP(hipStreamSynchronize(s));
     const int32_t *t = p->h_tally;
     if (stats) {
@@ -1141,17 +873,20 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
         stats->tasks = t[T_NTASK] - t[T_NPAD];
         stats->wide_tasks = t[T_NLARGE];
         stats->unsupported = t[T_UNSUP];
+        stats->fp64_repriced = t[T_NREDO];
     }
     if (t[T_INVALID] > 0) return fail(HWT_EINVAL, std::to_string(t[T_INVALID]) + " option(s) with invalid parameters (price NaN)");
     if (t[T_UNSUP] > 0)
         return fail(HWT_EUNSUPPORTED, std::to_string(t[T_UNSUP]) + " option(s) wider than " +
-                                          std::to_string(WAVE_MAX_WIDTH) + " nodes (price NaN)");
+                                          std::to_string(p->cfg.form == HWT_FORM_LANE ? p->lane_width : WAVE_MAX_WIDTH) +
+                                          " nodes (price NaN)");
     return HWT_OK;
 }
 
 // Device copies of host SoA arrays, one allocation.
 struct DeviceBatch {
     void *mem = nullptr;
+    size_t bytes = 0;
     hwt_options_f64 o{};
     hwt_yield_f64 y{};
     double *out = nullptr;
@@ -1174,6 +909,7 @@ static int upload(const hwt_options_f64 *h, const hwt_yield_f64 *hy, DeviceBatch
                  o_ts = carve(n * 2), o_a = carve(n * 8), o_sig = carve(n * 8), o_type = carve(n),
                  o_rate = carve(ny * 8), o_day = carve(ny * 4), o_out = carve(need_out ? n * 8 : 0);
     if (hipMalloc(&d.mem, std::max<size_t>(off, 256)) != hipSuccess) return fail(HWT_ENOMEM, "hipMalloc batch failed");
+    d.bytes = std::max<size_t>(off, 256);
     char *b = (char *)d.mem;
     auto put = [&](size_t at, const void *src, size_t bytes) {
         return bytes ? hipMemcpy(b + at, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
@@ -1208,6 +944,238 @@ struct DeviceGuard {
     }
 };
 
+} // extern "C"
+
+// Widest tree (2*jmax+1) among the valid options of host arrays, from the
+// same constant formulas as the device (0 when none is valid).
+static int host_max_width(const hwt_options_f64 *o)
+{
+    int w = 0;
+    for (int i = 0; i < o->n; ++i) {
+        OptConst c;
+        if (option_constants(o->strike[i], o->maturity[i], o->length[i], o->term_unit[i], o->term_steps[i], o->a[i],
+                             o->sigma[i], o->type[i], 1 << 30, c) == ST_OK)
+            w = std::max(w, (int)c.w);
+    }
+    return w;
+}
+
+// One execution of a fresh plan over device views, results copied to host.
+static int price_views_once(const DevOpts &o, const DevYield &y, double *dev_out, double *host_out, hwt_config c,
+                            hwt_stats *stats)
+{
+    c.flags |= HWT_FLAG_NO_GRAPH;  // one execution: a graph would not be replayed
+    hwt_plan *p = nullptr;
+    int rc = hwt_plan_create(&p, o.n, y.n, &c);
+    if (rc) return rc;
+    rc = plan_execute_views(p, o, y, dev_out, p->stream);
+    int rs = rc ? rc : hwt_plan_sync(p, stats);
+    const std::string keep = g_last_error;
+    if (rc == HWT_OK) {
+        if (hipMemcpy(host_out, dev_out, (size_t)o.n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rs = fail(HWT_EHIP, "hipMemcpy D2H failed");
+        else
+            g_last_error = keep;
+    }
+    hwt_plan_destroy(p);
+    return rs;
+}
+
+// A batch resident in device memory: raw options (DeviceBatch), or per-option
+// tree constants (the constants entry points and FP32 callers), with the
+// curve and an output array.
+struct hwt_batch {
+    int device = 0;
+    int max_width = 0;  // widest valid tree, for form 1's column buffers
+    DeviceBatch d;      // raw options
+    void *cmem = nullptr;  // constants, curve and output, one allocation
+    DevOpts o{};
+    DevYield y{};
+    double *out = nullptr;
+    size_t bytes = 0;
+    ~hwt_batch()
+    {
+        if (cmem) (void)hipFree(cmem);
+    }
+};
+
+// A batch of constants: k[0..n) and the curve copied to the current device.
+static int batch_of_constants(hwt_batch *b, const hwt_option_constants_f64 *k, int32_t n, const hwt_yield_f64 *yield,
+                              bool f32)
+{
+    const size_t ny = (size_t)yield->n;
+    auto part = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t cb = part((size_t)n * sizeof(*k)), rb = part(ny * 8), db = part(ny * 4);
+    const size_t bytes = cb + rb + db + part((size_t)n * 8);
+    if (hipMalloc(&b->cmem, bytes) != hipSuccess) return fail(HWT_ENOMEM, "hipMalloc constants batch failed");
+    b->bytes = bytes;
+    char *base = (char *)b->cmem;
+    auto *dc = (hwt_option_constants_f64 *)base;
+    auto *dr = (double *)(base + cb);
+    auto *dd = (int32_t *)(base + cb + rb);
+    b->out = (double *)(base + cb + rb + db);
+    if ((n > 0 && hipMemcpy(dc, k, (size_t)n * sizeof(*k), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(dr, yield->rate, ny * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dd, yield->day, ny * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(HWT_EHIP, "hipMemcpy H2D failed");
+    b->o = DevOpts{};
+    b->o.n = n;
+    b->o.consts = dc;
+    b->o.consts_f32 = f32 ? 1 : 0;
+    b->y = DevYield{yield->n, dr, dd};
+    int w = 3;
+    for (int i = 0; i < n; ++i) w = std::max(w, std::min(k[i].width, LANE_MAX_WIDTH + 1));
+    b->max_width = w;
+    return HWT_OK;
+}
+
+static int new_batch(hwt_batch **out, hwt_batch *&b)
+{
+    *out = nullptr;
+    b = new (std::nothrow) hwt_batch();
+    if (!b) return fail(HWT_ENOMEM, "hwt_batch: host allocation");
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        delete b;
+        b = nullptr;
+        return fail(HWT_EHIP, "hipGetDevice failed");
+    }
+    return HWT_OK;
+}
+
+// The tree constants of an option as the reference's default build (real =
+// float) derives them, common/OptionConstants.hpp:29-53: float arithmetic,
+// except that the unqualified exp, sqrt and ceil there are the C library's
+// double functions (<cmath> declares only those in the global namespace),
+// their results rounded to float where the float fields store them.
+// Invalid parameters get width 0, which the constants path rejects.
+static hwt_option_constants_f64 constants_f32(float strike, float T, float len, uint16_t tu, uint16_t steps, float a,
+                                              float sigma, int8_t type)
+{
+    hwt_option_constants_f64 k{};
+    k.t = len;
+    k.X = strike;
+    k.term_unit = tu;
+    k.type = type;
+    if ((type != 0 && type != 1) || tu == 0 || steps == 0 || !(a > 0.0f) || !(sigma >= 0.0f) || !(T >= 0.0f) ||
+        !(strike == strike))
+        return k;
+    const int tuy = (int)std::ceil((double)(365.0f / (float)tu));
+    const float nf = (float)((int)steps * tuy) * T;  // int * int, then * float
+    if (!(nf < 2147483647.0f)) return k;
+    const float dt = (float)tuy / (float)steps;
+    const double V = (double)(sigma * sigma) * (1.0 - std::exp((double)(-2.0f * a * dt))) / (double)(2.0f * a);
+    const float dr = (float)std::sqrt(3.0 * V);
+    const float M = (float)(std::exp((double)(-a * dt)) - 1.0);
+    const float q = -0.184f / M;
+    if (!(q < 1.0e9f)) return k;
+    k.n = (int32_t)nf;
+    k.dt = dt;
+    k.dr = dr;
+    k.M = M;
+    k.jmax = (int32_t)q + 1;
+    k.width = 2 * k.jmax + 1;
+    return k;
+}
+
+static bool valid_opts_f32(const hwt_options_f32 *o)
+{
+    return o && o->n >= 0 &&
+           (o->n == 0 || (o->strike && o->maturity && o->length && o->term_unit && o->term_steps && o->a &&
+                          o->sigma && o->type));
+}
+
+static bool valid_yield_f32(const hwt_yield_f32 *y) { return y && y->n > 0 && y->rate && y->day; }
+
+// The FP32 build's constants of every option of host float arrays, and the
+// curve's rates widened (float -> double is exact).
+static int widen_f32(const hwt_options_f32 *opts, const hwt_yield_f32 *yield, std::vector<hwt_option_constants_f64> &k,
+                     std::vector<double> &rates)
+{
+    try {
+        k.resize((size_t)opts->n);
+        rates.assign(yield->rate, yield->rate + yield->n);
+    } catch (const std::bad_alloc &) {
+        return fail(HWT_ENOMEM, "host allocation");
+    }
+    for (int i = 0; i < opts->n; ++i)
+        k[i] = constants_f32(opts->strike[i], opts->maturity[i], opts->length[i], opts->term_unit[i],
+                             opts->term_steps[i], opts->a[i], opts->sigma[i], opts->type[i]);
+    return HWT_OK;
+}
+
+extern "C" {
+
+int hwt_batch_create(hwt_batch **out, const hwt_options_f64 *opts, const hwt_yield_f64 *yield)
+{
+    g_last_error.clear();
+    if (!out || !valid_opts_ptrs(opts) || !valid_yield_ptrs(yield))
+        return fail(HWT_EINVAL, "hwt_batch_create: null or negative-size argument");
+    hwt_batch *b = nullptr;
+    int rc = new_batch(out, b);
+    if (rc) return rc;
+    rc = upload(opts, yield, b->d, true);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    b->bytes = b->d.bytes;
+    b->o = dev_opts(b->d.o);
+    b->y = DevYield{b->d.y.n, b->d.y.rate, b->d.y.day};
+    b->out = b->d.out;
+    b->max_width = host_max_width(opts);
+    *out = b;
+    return HWT_OK;
+}
+
+int hwt_batch_create_f32(hwt_batch **out, const hwt_options_f32 *opts, const hwt_yield_f32 *yield)
+{
+    g_last_error.clear();
+    if (!out || !valid_opts_f32(opts) || !valid_yield_f32(yield))
+        return fail(HWT_EINVAL, "hwt_batch_create_f32: null or negative-size argument");
+    std::vector<hwt_option_constants_f64> k;
+    std::vector<double> rates;
+    int rc = widen_f32(opts, yield, k, rates);
+    if (rc) return rc;
+    hwt_batch *b = nullptr;
+    rc = new_batch(out, b);
+    if (rc) return rc;
+    const hwt_yield_f64 y{yield->n, rates.data(), yield->day};
+    rc = batch_of_constants(b, k.data(), opts->n, &y, true);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return HWT_OK;
+}
+
+int64_t hwt_batch_device_memory(const hwt_batch *b) { return b ? (int64_t)b->bytes : 0; }
+
+void hwt_batch_destroy(hwt_batch *b)
+{
+    if (!b) return;
+    int prev = -1;
+    const bool swap = hipGetDevice(&prev) == hipSuccess && prev != b->device && hipSetDevice(b->device) == hipSuccess;
+    delete b;
+    if (swap) (void)hipSetDevice(prev);
+}
+
+int hwt_batch_price(hwt_batch *b, double *host_out, const hwt_config *cfg, hwt_stats *stats)
+{
+    g_last_error.clear();
+    if (!b || (b->o.n > 0 && !host_out)) return fail(HWT_EINVAL, "hwt_batch_price: null argument");
+    hwt_config c;
+    if (cfg) c = *cfg;
+    else hwt_config_init(&c);
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (b->o.n == 0) return HWT_OK;
+    DeviceGuard g;
+    int rc = g.enter(b->device);
+    if (rc) return rc;
+    if (c.lane_max_width <= 0) c.lane_max_width = std::max(b->max_width, 3);
+    return price_views_once(b->o, b->y, b->out, host_out, c, stats);
+}
+
 int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, double *out_prices, const hwt_config *cfg,
                   hwt_stats *stats)
 {
@@ -1222,24 +1190,88 @@ int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, doubl
     if (rc) return rc;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (opts->n == 0) return HWT_OK;
-    DeviceBatch d;
-    rc = upload(opts, yield, d, true);
+    hwt_batch *b = nullptr;
+    rc = hwt_batch_create(&b, opts, yield);
     if (rc) return rc;
-    c.flags |= HWT_FLAG_NO_GRAPH;  // one execution: a graph would not be replayed
-    hwt_plan *p = nullptr;
-    rc = hwt_plan_create(&p, opts->n, yield->n, &c);
+    rc = hwt_batch_price(b, out_prices, &c, stats);
+    hwt_batch_destroy(b);
+    return rc;
+}
+
+} // extern "C"
+
+// One-shot pricing of a batch of constants (host arrays) into host_out.
+static int price_constants(const hwt_option_constants_f64 *consts, int32_t n, const hwt_yield_f64 *yield,
+                           double *host_out, const hwt_config *cfg, hwt_stats *stats, bool f32)
+{
+    hwt_config c;
+    if (cfg) c = *cfg;
+    else hwt_config_init(&c);
+    DeviceGuard g;
+    int rc = g.enter(c.device);
     if (rc) return rc;
-    rc = hwt_plan_execute(p, &d.o, &d.y, d.out, nullptr);
-    int rs = rc ? rc : hwt_plan_sync(p, stats);
-    const std::string keep = g_last_error;
-    if (rc == HWT_OK) {
-        if (hipMemcpy(out_prices, d.out, (size_t)opts->n * 8, hipMemcpyDeviceToHost) != hipSuccess)
-            rs = fail(HWT_EHIP, "hipMemcpy D2H failed");
-        else
-            g_last_error = keep;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (n == 0) return HWT_OK;
+    hwt_batch *b = nullptr;
+    rc = new_batch(&b, b);
+    if (rc) return rc;
+    rc = batch_of_constants(b, consts, n, yield, f32);
+    if (rc == HWT_OK) rc = hwt_batch_price(b, host_out, &c, stats);
+    hwt_batch_destroy(b);
+    return rc;
+}
+
+extern "C" {
+
+int hwt_price_f32(const hwt_options_f32 *opts, const hwt_yield_f32 *yield, float *out_prices, const hwt_config *cfg,
+                  hwt_stats *stats)
+{
+    g_last_error.clear();
+    if (!valid_opts_f32(opts) || !valid_yield_f32(yield) || (opts->n > 0 && !out_prices))
+        return fail(HWT_EINVAL, "hwt_price_f32: null or negative-size argument");
+    std::vector<hwt_option_constants_f64> k;
+    std::vector<double> rates, prices;
+    int rc = widen_f32(opts, yield, k, rates);
+    if (rc) return rc;
+    try {
+        prices.resize((size_t)opts->n);
+    } catch (const std::bad_alloc &) {
+        return fail(HWT_ENOMEM, "hwt_price_f32: host allocation");
     }
-    hwt_plan_destroy(p);
-    return rs;
+    const hwt_yield_f64 y{yield->n, rates.data(), yield->day};
+    rc = price_constants(k.data(), opts->n, &y, prices.data(), cfg, stats, true);
+    if (rc == HWT_OK || rc == HWT_EUNSUPPORTED || rc == HWT_EINVAL)  // NaN rows are part of the output
+        for (int i = 0; i < opts->n; ++i) out_prices[i] = (float)prices[i];
+    return rc;
+}
+
+int hwt_price_constants_f64(const hwt_option_constants_f64 *consts, int32_t n, const hwt_yield_f64 *yield,
+                            double *out_prices, const hwt_config *cfg, hwt_stats *stats)
+{
+    g_last_error.clear();
+    if (n < 0 || (n > 0 && (!consts || !out_prices)) || !valid_yield_ptrs(yield))
+        return fail(HWT_EINVAL, "hwt_price_constants_f64: null or negative-size argument");
+    return price_constants(consts, n, yield, out_prices, cfg, stats, false);
+}
+
+int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, const hwt_yield_f32 *yield,
+                            float *out_prices, const hwt_config *cfg, hwt_stats *stats)
+{
+    g_last_error.clear();
+    if (n < 0 || (n > 0 && (!consts || !out_prices)) || !valid_yield_f32(yield))
+        return fail(HWT_EINVAL, "hwt_price_constants_f32: null or negative-size argument");
+    std::vector<hwt_option_constants_f64> wide((size_t)n);
+    for (int i = 0; i < n; ++i) {  // float -> double is exact
+        const hwt_option_constants_f32 &k = consts[i];
+        wide[i] = hwt_option_constants_f64{k.t, k.dt, k.dr, k.X, k.M, k.jmax, k.n, k.width, k.term_unit, k.type};
+    }
+    const std::vector<double> rates(yield->rate, yield->rate + yield->n);
+    const hwt_yield_f64 y{yield->n, rates.data(), yield->day};
+    std::vector<double> prices((size_t)n);
+    const int rc = price_constants(wide.data(), n, &y, prices.data(), cfg, stats, true);
+    if (rc == HWT_OK || rc == HWT_EUNSUPPORTED || rc == HWT_EINVAL)
+        for (int i = 0; i < n; ++i) out_prices[i] = (float)prices[i];
+    return rc;
 }
 
 int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, int64_t *offsets, double *alphas)

@@ -631,7 +631,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (k + 1 < g.y.n && g.y.day[k] > g.y.day[k + 1]) unsorted = 1;
         }
     }
-    const bool sorted = staged && !__syncthreads_or(unsorted);
+    // unconditional barrier: it also orders the LDS writes below after every
+    // thread's curve staging, whatever the curve's length
+    const int any_unsorted = __syncthreads_or(unsorted);
+    const bool sorted = staged && !any_unsorted;
     if (t < INF_ROWS) {
         const int i = r0 + t < nrows ? (g.rows ? g.rows[r0 + t] : r0 + t) : -1;
         float x[NIN] = {};
@@ -741,7 +744,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (g.mean) g.mean[i] = ok ? mean : NAN;
             if (g.spread) g.spread[i] = ok ? spread : NAN;
             if (g.in_box) g.in_box[i] = (int8_t)ok;
-            const bool accept = ok && spread <= (double)g.threshold;
+            // a non-finite member output makes the mean non-finite while
+            // fminf/fmaxf would drop it from the spread: never accepted
+            const bool accept = ok && isfinite(mean) && spread <= (double)g.threshold;
             if (g.out && accept) g.out[i] = mean;
             if (g.fb_idx && !accept) {
                 fb_local[atomicAdd(&fb_n, 1)] = i;
@@ -1332,6 +1337,11 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
         return fail(HWT_EINVAL, "hwt_surr_price: null or negative-size argument");
     if (o->n > s->infer_cap) return fail(HWT_ENOMEM, "hwt_surr_price: more options than the capacity");
     if (y->n > s->ycap) return fail(HWT_ENOMEM, "hwt_surr_price: yield curve longer than the capacity");
+    // one member has no spread (always 0): a finite threshold would accept
+    // every in-box price with no error estimate at all
+    if (s->members < 2 && std::isfinite(s->cfg.spread_threshold))
+        return fail(HWT_EINVAL, "hwt_surr_price: a one-member ensemble has no spread to gate on; "
+                                "set spread_threshold to +inf to take every in-box price from the network");
     DeviceGuardS dg(s->device);
     hipStream_t st = s->stream;
     SURR_HIP(hipEventRecord(s->e0, st));

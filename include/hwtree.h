@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define HWT_ABI_VERSION 1
+#define HWT_ABI_VERSION 2
 
 /* Return codes. */
 #define HWT_OK            0
@@ -93,6 +93,9 @@ typedef struct {
     int32_t wide_per_cu;   /* form 2: resident workgroups per CU of the persistent
                               wide-tree kernel; 0 = as many as fit */
     int32_t narrow_per_cu; /* form 2: the same for the narrow-tree kernel */
+    int32_t lane_max_width;/* form 1: the widest tree the plan prices (its tree columns in
+                              HBM are sized for it; wider options are unsupported);
+                              0 = hwt_max_width.  The one-shot calls set it from the batch. */
 } hwt_config;
 
 /* Launch the plan's kernels one by one instead of replaying its HIP graph;
@@ -123,7 +126,9 @@ typedef struct {
     int32_t tasks;            /* work items scheduled */
     int32_t wide_tasks;       /* ... of which wide trees */
     int32_t unsupported;      /* options that could not be priced (output NaN) */
-    int32_t reserved;
+    int32_t fp64_repriced;    /* HWT_PREC_MIXED: options whose FP32 tree state left FP32's
+                                 range (long, weakly mean-reverting bonds) and were priced
+                                 again with FP64 states in the same execution */
 } hwt_stats;
 
 /* Fills *cfg with defaults (auto form, FP64, tuned sizes, work-descending order). */
@@ -149,6 +154,93 @@ int  hwt_plan_execute(hwt_plan *plan, const hwt_options_f64 *dev_opts,
  * option could not be priced) and fills stats if non-NULL. */
 int  hwt_plan_sync(hwt_plan *plan, hwt_stats *stats);
 void hwt_plan_destroy(hwt_plan *plan);
+
+/* trinom::Options and trinom::Yield as a default-built reference caller holds
+ * them: `real` is float unless USE_DOUBLE (common/Real.hpp:17-21). */
+typedef struct {
+    int32_t n;
+    const float    *strike;
+    const float    *maturity;
+    const float    *length;
+    const uint16_t *term_unit;
+    const uint16_t *term_steps;
+    const float    *a;
+    const float    *sigma;
+    const int8_t   *type;
+} hwt_options_f32;
+
+typedef struct {
+    int32_t n;
+    const float   *rate;
+    const int32_t *day;
+} hwt_yield_f32;
+
+/* One-shot pricing for FP32 callers; replaces seq::computeOptions and the
+ * KernelRunBase::run overloads of the reference's default FP32 build
+ * (seq/Seq.hpp:181-190, cuda-option/Kernel.cuh:303-314,
+ * cuda-multi/Kernel.cuh:374-403, all with real = float).  Each option's tree
+ * is the one that build derives: its constants (height, dt, dr, M, jmax) as
+ * common/OptionConstants.hpp:29-53 computes them in float, and the exercise
+ * step (int)(t / dt) in float (seq/Seq.hpp:125), which for some step counts
+ * is one step earlier than in FP64 (15 steps a year at a 3-year expiry: 44,
+ * not 45).  That tree is priced as hwt_price_f64 prices (cfg->precision:
+ * HWT_PREC_FP64 by default, or HWT_PREC_MIXED), on the curve's rates widened
+ * exactly, and each price is rounded once to float. */
+int hwt_price_f32(const hwt_options_f32 *opts, const hwt_yield_f32 *yield, float *out_prices,
+                  const hwt_config *cfg, hwt_stats *stats);
+
+/* trinom::OptionConstants (common/OptionConstants.hpp:14-53): an option's
+ * tree constants as the reference derives them from an Options row.  Pricing
+ * from constants replaces seq::computeSingleOption(const OptionConstants&,
+ * const Yield&) (seq/Seq.hpp:24): the exercise step is (int)(t / dt) as in
+ * seq/Seq.hpp:125, the tree is width = 2*jmax+1 nodes wide and n steps high,
+ * and M, dr and dt are taken as given. */
+typedef struct {
+    double   t;          /* option expiry in years (the Length field) */
+    double   dt;         /* years per step */
+    double   dr;         /* rate spacing of the tree */
+    double   X;          /* strike */
+    double   M;          /* e^{-a dt} - 1 */
+    int32_t  jmax;
+    int32_t  n;          /* height in steps */
+    int32_t  width;      /* must equal 2*jmax + 1 */
+    uint16_t term_unit;  /* days per term unit (yield lookups) */
+    int8_t   type;       /* CALL = 0, PUT = 1 */
+} hwt_option_constants_f64;
+
+/* Prices n options given by their constants (host arrays) into out_prices
+ * (host, input order).  cfg and stats may be NULL. */
+int hwt_price_constants_f64(const hwt_option_constants_f64 *consts, int32_t n, const hwt_yield_f64 *yield,
+                            double *out_prices, const hwt_config *cfg, hwt_stats *stats);
+
+/* The same for constants a default-built (real = float) caller derived: the
+ * exercise step is (int)(t / dt) evaluated in float, as the FP32 reference
+ * evaluates it; the tree is then priced as hwt_price_f32 prices. */
+typedef struct {
+    float    t, dt, dr, X, M;
+    int32_t  jmax, n, width;
+    uint16_t term_unit;
+    int8_t   type;
+} hwt_option_constants_f32;
+
+int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, const hwt_yield_f32 *yield,
+                            float *out_prices, const hwt_config *cfg, hwt_stats *stats);
+
+/* A batch held in device memory across pricing calls: the upload of
+ * cuda::CudaOptions (cuda/CudaDomain.cuh:88-135, constructed once, then passed
+ * to KernelRunBase::run(CudaOptions&, ...), cuda-option/Kernel.cuh:316-325).
+ * hwt_batch_create copies the host arrays to the current device;
+ * hwt_batch_price prices the resident batch (a plan per call, no re-upload)
+ * into host_out (input order). */
+typedef struct hwt_batch hwt_batch;
+int hwt_batch_create(hwt_batch **batch, const hwt_options_f64 *host_opts, const hwt_yield_f64 *host_yield);
+/* The same for a real = float caller's arrays: the batch holds each option's
+ * tree as the FP32 build derives it (see hwt_price_f32). */
+int hwt_batch_create_f32(hwt_batch **batch, const hwt_options_f32 *host_opts, const hwt_yield_f32 *host_yield);
+int hwt_batch_price(hwt_batch *batch, double *host_out, const hwt_config *cfg, hwt_stats *stats);
+/* Bytes of device memory the batch holds (inputs and output). */
+int64_t hwt_batch_device_memory(const hwt_batch *batch);
+void hwt_batch_destroy(hwt_batch *batch);
 
 /* Per-step alpha calibration of each option (alphas[0..n] of the reference's
  * forward pass, seq/Seq.hpp:56-106), host arrays.  alpha_offsets (n_opts+1

@@ -17,6 +17,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 REF_SEQ = HERE / "_ref" / "Seq"
+REF_SEQ32 = HERE / "_ref" / "Seq32"
 
 
 class Constants(C.Structure):
@@ -52,6 +53,10 @@ def lib():
         L.orc_call_value.restype = C.c_double
         L.orc_price_single.argtypes = [C.POINTER(Constants), d, i32, C.c_int, d]
         L.orc_price_single.restype = C.c_double
+        L.orc_option_constants_f32.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint16, C.c_uint16,
+                                               C.c_float, C.c_float, C.c_int8, C.POINTER(Constants)]
+        L.orc_price_single_f32step.argtypes = [C.POINTER(Constants), d, i32, C.c_int]
+        L.orc_price_single_f32step.restype = C.c_double
         L.orc_price_batch.argtypes = [C.c_int, d, d, d, u16, u16, d, d, i8, d, i32, C.c_int, d, C.c_int]
         L.orc_node_updates.argtypes = [C.POINTER(Constants), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
@@ -94,7 +99,7 @@ def price(opts, y, threads: int | None = None) -> np.ndarray:
     """seq::computeOptions over the whole batch (seq/Seq.hpp:181-190)."""
     out = np.empty(opts.N, dtype=np.float64)
     if threads is None:
-        threads = os.cpu_count() or 1
+        threads = min(16, os.cpu_count() or 1)
     lib().orc_price_batch(opts.N, _p(opts.strike, C.c_double), _p(opts.maturity, C.c_double),
                           _p(opts.length, C.c_double), _p(opts.term_unit, C.c_uint16),
                           _p(opts.term_steps, C.c_uint16), _p(opts.a, C.c_double),
@@ -102,6 +107,41 @@ def price(opts, y, threads: int | None = None) -> np.ndarray:
                           _p(y.rates, C.c_double), _p(y.days, C.c_int32), y.N,
                           _p(out, C.c_double), int(threads))
     return out
+
+
+def constants_f32(opts, i: int) -> Constants:
+    """The tree constants a real = float build of the reference derives
+    (common/OptionConstants.hpp:29-53 in float), widened."""
+    c = Constants()
+    f = np.float32
+    lib().orc_option_constants_f32(f(opts.strike[i]), f(opts.maturity[i]), f(opts.length[i]),
+                                   int(opts.term_unit[i]), int(opts.term_steps[i]), f(opts.a[i]),
+                                   f(opts.sigma[i]), int(opts.type[i]), C.byref(c))
+    return c
+
+
+def price_f32shape(opts, y) -> np.ndarray:
+    """The reference's FP32 build's trees (float constants, exercise step in
+    float) evaluated by seq's algorithm in double, on the float-rounded
+    curve: what an FP32 caller's prices are without FP32 rounding."""
+    rates = np.ascontiguousarray(y.rates.astype(np.float32).astype(np.float64))
+    days = np.ascontiguousarray(y.days, dtype=np.int32)
+    out = np.empty(opts.N)
+    for i in range(opts.N):
+        c = constants_f32(opts, i)
+        out[i] = lib().orc_price_single_f32step(C.byref(c), _p(rates, C.c_double), _p(days, C.c_int32), y.N)
+    return out
+
+
+def run_reference_seq32(options_path, yield_path) -> list[str] | None:
+    """The reference's own seq built with its default real = float
+    (oracle/_ref/Seq32); None when that binary was not built here."""
+    if not REF_SEQ32.exists():
+        return None
+    out = subprocess.run([str(REF_SEQ32), "-o", str(options_path), "-y", str(yield_path)],
+                         check=True, capture_output=True, text=True).stdout
+    line = out.strip().splitlines()[0]
+    return [t.strip() for t in line.strip()[1:-1].split(",")]
 
 
 def node_updates(opts, i: int):

@@ -49,6 +49,36 @@ void orc_option_constants(double strike, double maturity, double length,
     c->width = 2 * c->jmax + 1;
 }
 
+/* common/OptionConstants.hpp:29-53 in the reference's default build, where
+ * real = float (common/Real.hpp:17-21): float arithmetic, except that the
+ * unqualified exp, sqrt and ceil there resolve to the C library's double
+ * functions (<cmath> declares only those in the global namespace), whose
+ * results are rounded to float where they are stored in the float fields. */
+void orc_option_constants_f32(float strike, float maturity, float length,
+                              uint16_t termUnit, uint16_t termStepCount,
+                              float a, float sigma, int8_t type,
+                              orc_constants *c)
+{
+    c->termUnit = termUnit;
+    const float T = maturity;
+    const int termUnitsInYearCount = (int)ceil((double)((float)R_YEAR / (float)termUnit));
+    /* uint16 * int -> int, then * float, truncated on assignment to int */
+    c->n = (int32_t)((float)((int)termStepCount * termUnitsInYearCount) * T);
+    c->t = length;
+    const float dt = (float)termUnitsInYearCount / (float)termStepCount;
+    c->dt = dt;
+    c->type = type;
+    c->X = strike;
+    /* sigma*sigma (float) * (one - exp(...)) (double) / (two*a) (float) */
+    const double V = (double)(sigma * sigma) * (1.0 - exp((double)(-2.0f * a * dt))) / (double)(2.0f * a);
+    const float dr = (float)sqrt(3.0 * V);
+    const float M = (float)(exp((double)(-a * dt)) - 1.0);
+    c->dr = dr;
+    c->M = M;
+    c->jmax = (int32_t)(-0.184f / M) + 1;
+    c->width = 2 * c->jmax + 1;
+}
+
 /* common/Domain.hpp:11-46 */
 double orc_yield_at_year(double t, int termUnit, const double *prices,
                          const int32_t *timeSteps, int size)
@@ -132,9 +162,27 @@ typedef struct {
     double rate, pu, pm, pd;
 } jvalue;
 
+static double price_single(const orc_constants *c, const double *yprices, const int32_t *ydays, int ysize,
+                           double *alphas_out, int f32_step);
+
 /* seq/Seq.hpp:24-179 */
 double orc_price_single(const orc_constants *c, const double *yprices,
                         const int32_t *ydays, int ysize, double *alphas_out)
+{
+    return price_single(c, yprices, ydays, ysize, alphas_out, 0);
+}
+
+/* seq/Seq.hpp:24-179 on the tree of a real = float build's constants
+ * (orc_option_constants_f32): the exercise test i == (int)(t / dt) of
+ * seq/Seq.hpp:125 is then evaluated in float; everything else in double. */
+double orc_price_single_f32step(const orc_constants *c, const double *yprices,
+                                const int32_t *ydays, int ysize)
+{
+    return price_single(c, yprices, ydays, ysize, NULL, 1);
+}
+
+static double price_single(const orc_constants *c, const double *yprices, const int32_t *ydays, int ysize,
+                           double *alphas_out, int f32_step)
 {
     const int width = c->width, jmax = c->jmax, n = c->n;
     const int jmin = -jmax;
@@ -205,7 +253,7 @@ double orc_price_single(const orc_constants *c, const double *yprices,
     for (int i = n - 1; i >= 0; --i) {
         const int jhigh = i < jmax ? i : jmax;
         const double alpha = alphas[i];
-        const int isMaturity = i == ((int)(c->t / c->dt));
+        const int isMaturity = i == (f32_step ? (int)((float)c->t / (float)c->dt) : (int)(c->t / c->dt));
         for (int j = -jhigh; j <= jhigh; ++j) {
             const int jind = j - jmin;
             const jvalue jval = jv[jind];

@@ -19,6 +19,10 @@
  *                           thesis book KATs, ZCB identity, put-call parity,
  *                           the compiled reference binary
  *   orc_alphas           -- sum_j Q_{m+1}[j] e^{-j dr dt} = P((m+2) dt) identity
+ *   orc_option_constants_f32, orc_price_single_f32step
+ *                        -- the reference's FP32 golden files data/out32 and
+ *                           its FP32 seq binary (oracle/_ref/Seq32), at the
+ *                           reference's FP32 comparison (test/Compare.hpp)
  */
 #ifndef HWTREE_SEQ_ORACLE_H
 #define HWTREE_SEQ_ORACLE_H
@@ -49,6 +53,14 @@ void orc_option_constants(double strike, double maturity, double length,
                           double a, double sigma, int8_t type,
                           orc_constants *out);
 
+/* The same constants as the reference's default build (real = float)
+ * derives them: float arithmetic with the C library's double exp, sqrt and
+ * ceil, stored widened. */
+void orc_option_constants_f32(float strike, float maturity, float length,
+                              uint16_t termUnit, uint16_t termStepCount,
+                              float a, float sigma, int8_t type,
+                              orc_constants *out);
+
 /* common/Domain.hpp:11-46 */
 double orc_yield_at_year(double t, int termUnit, const double *prices,
                          const int32_t *timeSteps, int size);
@@ -67,6 +79,11 @@ double orc_call_value(int isMaturity, double X, int8_t type, double res);
 /* seq/Seq.hpp:24-179.  alphas_out (nullable) receives alphas[0..n]. */
 double orc_price_single(const orc_constants *c, const double *yprices,
                         const int32_t *ydays, int ysize, double *alphas_out);
+
+/* seq/Seq.hpp:24-179 evaluated in double on the tree of float-build
+ * constants, with the exercise step (int)(t / dt) taken in float. */
+double orc_price_single_f32step(const orc_constants *c, const double *yprices,
+                                const int32_t *ydays, int ysize);
 
 /* seq/Seq.hpp:181-190 (OpenMP parallel-for over options when threads > 1). */
 void orc_price_batch(int n, const double *strike, const double *maturity,

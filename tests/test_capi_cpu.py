@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(pkg):
 
 def test_host_only_entry_points(pkg):
     lib = pkg.capi.lib()
-    assert lib.hwt_abi_version() == 1
+    assert lib.hwt_abi_version() == 2
     assert lib.hwt_max_width(pkg.capi.FORM_WAVE) == 1344
     assert lib.hwt_strerror(0) == b"success"
     assert lib.hwt_strerror(-1) == b"invalid argument"
@@ -49,7 +49,9 @@ def test_struct_layouts_match_header(pkg):
     # 8-byte aligned pointers after an int32 count
     assert C.sizeof(pkg.capi.OptionsC) == 8 + 8 * 8
     assert C.sizeof(pkg.capi.YieldC) == 24
-    assert C.sizeof(pkg.capi.ConfigC) == 8 * 4
+    assert C.sizeof(pkg.capi.ConfigC) == 9 * 4
+    assert C.sizeof(pkg.capi.ConstantsC) == 5 * 8 + 3 * 4 + 2 + 1 + 1  # trailing pad
+    assert C.sizeof(pkg.capi.ConstantsF32C) == 5 * 4 + 3 * 4 + 2 + 1 + 1
     assert C.sizeof(pkg.capi.TuneReportC) == 3 * 8 + 2 * 4
     text = (ROOT / "include" / "hwtree.h").read_text()
     for name, _ in pkg.capi.ConfigC._fields_:
@@ -79,6 +81,14 @@ def test_null_arguments_rejected_without_gpu(pkg):
     lib = pkg.capi.lib()
     assert lib.hwt_price_f64(None, None, None, None, None) == pkg.capi.HWT_EINVAL
     assert lib.hwt_plan_create(None, 10, 10, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_price_f32(None, None, None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_price_constants_f64(None, 1, None, None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_price_constants_f32(None, 1, None, None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_batch_create(None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_batch_create_f32(None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_batch_price(None, None, None, None) == pkg.capi.HWT_EINVAL
+    assert lib.hwt_batch_device_memory(None) == 0
+    lib.hwt_batch_destroy(None)  # no-op
     assert lib.hwt_plan_sync(None, None) == pkg.capi.HWT_EINVAL
     assert lib.hwt_autotune(None, None, None, None, None) == pkg.capi.HWT_EINVAL
     lib.hwt_plan_destroy(None)  # no-op

@@ -9,6 +9,16 @@ from conftest import GOLDEN, ROOT, parity_tol
 
 BIN = ROOT / "diku.optionspricing_amd" / "bin"
 TEST_BOOK = ROOT / "tests" / "cpp" / "test_book"
+TEST_BOOK32 = ROOT / "tests" / "cpp" / "test_book32"
+FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+def catch_approx_ok(got, want):
+    """The reference's FP32 comparison (test/Compare.hpp:8-16): Catch's Approx
+    with margin 1000 * FLT_EPSILON, which also passes a difference within
+    100 * FLT_EPSILON of (1 + the larger magnitude)."""
+    d = np.abs(got - want)
+    return (d <= 1000 * FLT_EPS) | (d <= 100 * FLT_EPS * (1.0 + np.maximum(np.abs(got), np.abs(want))))
 
 
 def _need(p):
@@ -30,9 +40,13 @@ def test_cli_bad_flag_is_an_error():
 
 
 @pytest.mark.gpu
-def test_cpp_book_options_all_runners():
-    _need(TEST_BOOK)
-    r = subprocess.run([str(TEST_BOOK), str(GOLDEN / "yield.in")], capture_output=True, text=True, timeout=120)
+@pytest.mark.parametrize("exe", [TEST_BOOK, TEST_BOOK32])
+def test_cpp_book_options_all_runners(exe):
+    """Every C++ drop-in (both widths of `real`), including
+    seq::computeSingleOption on OptionConstants and the run(CudaOptions&)
+    overload, against the oracle."""
+    _need(exe)
+    r = subprocess.run([str(exe), str(GOLDEN / "yield.in")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK: 0 failure(s)" in r.stdout
 
@@ -50,6 +64,25 @@ def test_cli_matches_golden_file(pkg, exe, extra):
     got = np.array([float(t) for t in r.stdout.strip()[1:-1].split(",")])
     gold = pkg.formats.read_results(GOLDEN / "4_SKEWED_s64.out")
     assert np.all(np.abs(got - gold) <= parity_tol(gold) + 1e-11)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe,extra", [("hwt-seq32", []), ("hwt-multi32", ["-v", "3", "-s", "H"])])
+@pytest.mark.parametrize("name", ["book", "1_RAND_1000", "4_SKEWED_s64"])
+def test_cli32_matches_fp32_golden_file(pkg, exe, extra, name):
+    """The FP32 builds of the drivers against the reference's FP32 seq output
+    (data/out32), at the reference's own FP32 bar; the printed array has 9
+    decimals, as the reference prints floats."""
+    _need(BIN / exe)
+    r = subprocess.run([str(BIN / exe), "-o", str(GOLDEN / f"{name}.in"), "-y", str(GOLDEN / "yield.in"), *extra],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    toks = r.stdout.strip()[1:-1].split(",")
+    assert all(len(t.strip().split(".")[1]) == 9 for t in toks)
+    got = np.array([float(t) for t in toks])
+    gold = pkg.formats.read_results(GOLDEN / f"{name}.out32")
+    ok = catch_approx_ok(got, gold)
+    assert ok.all(), (np.nonzero(~ok)[0][:5], got[~ok][:5], gold[~ok][:5])
 
 
 @pytest.mark.gpu

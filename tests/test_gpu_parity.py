@@ -313,6 +313,28 @@ def test_yield_curve_shapes(pkg, oracle, curve):
     assert_parity(pkg.capi.price(opts, y), oracle.price(opts, y), curve)
 
 
+def test_long_curve_on_a_multi_block_batch(pkg, oracle):
+    """A 3,650-knot curve (more knots than the setup kernel stages in LDS) on
+    the 60,000-option batch, 235 setup blocks: the same piecewise-linear curve
+    as the 120-knot one, so every price, the task count and the work tallies
+    must equal the short curve's, and a stratified subset the oracle's.  The
+    setup kernel's per-block histogram and tallies are cleared in LDS before
+    any thread adds to them, for curves of every length."""
+    short = pkg.datasets.synthetic_yield()
+    days = np.arange(1, 3651, dtype=np.int32)  # every day to the short curve's last knot
+    dense = pkg.formats.Yield(np.interp(days, short.days, short.rates), days)
+    # knots of the short curve lie on the dense one: same interpolated rates
+    assert np.all(np.isin(short.days, days))
+    opts = pkg.datasets.skewed_60k()
+    got, st = pkg.capi.price(opts, dense, stats=True)
+    want, st0 = pkg.capi.price(opts, short, stats=True)
+    for k in ("tasks", "wide_tasks", "node_updates", "wide_node_updates", "seq_node_updates", "unsupported"):
+        assert st[k] == st0[k], k
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11)
+    sub = np.arange(0, opts.N, 97)
+    assert_parity(got[sub], oracle.price(opts.take(sub), dense), "3,650-knot curve")
+
+
 @pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"flags": 8}, {"flags": 10},
                                      {"flags": 9}, {"wide_per_cu": 2, "narrow_per_cu": 4},
                                      {"sort": "H"}, {"sort": "W"}])

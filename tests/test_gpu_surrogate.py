@@ -216,13 +216,16 @@ def test_inference_kernel_matches_cpu_reference(setup):
 
 @pytest.mark.parametrize("members", [1, 2])
 def test_smaller_ensembles_price_through_the_gate(pkg, setup, members):
-    """Ensembles of 1 and 2 members: a single member has spread 0, so every
-    in-box option takes the network's price; out-of-box ones still go to the
-    tree, and the network's prices equal predict()'s mean."""
+    """Ensembles of 1 and 2 members.  A single member has no spread (always
+    0), so it gates only with an infinite threshold, where every in-box option
+    takes the network's price; a finite threshold is refused.  Out-of-box
+    options still go to the tree, and the network's prices equal predict()'s
+    mean."""
     y = setup["y"]
     o = pkg.datasets.box_mix(51, 3000)
     dev = pkg.capi.to_device(o, y)
-    s = pkg.surrogate.Surrogate(0, 4096, y.N, pkg.surrogate.config(members=members, spread_threshold=1e-3))
+    thr = float("inf") if members == 1 else 1e-3
+    s = pkg.surrogate.Surrogate(0, 4096, y.N, pkg.surrogate.config(members=members, spread_threshold=thr))
     s.set_params(setup["s"].get_params()[:members])
     plan = pkg.surrogate.fallback_plan(4096, y.N)
     st = s.price(plan, dev)
@@ -238,6 +241,12 @@ def test_smaller_ensembles_price_through_the_gate(pkg, setup, members):
     tree = dev["out"].to_host()
     np.testing.assert_array_equal(got[~accepted], tree[~accepted])
     s.close()
+    if members == 1:
+        one = pkg.surrogate.Surrogate(0, 4096, y.N, pkg.surrogate.config(members=1, spread_threshold=1e-3))
+        one.set_params(setup["s"].get_params()[:1])
+        with pytest.raises(pkg.capi.HwtError, match="one-member"):
+            one.price(plan, dev)
+        one.close()
     plan.close()
 
 

@@ -1,0 +1,141 @@
+"""The two surrogate workloads at their full sizes.
+
+Training workload: 2,000,000 box samples priced by the FP64 tree, a 4-member
+ensemble (8 -> 256 x 4 -> 1, SiLU) trained with the project's BF16 MFMA
+forward/backward and FP32 Adam at batch 8,192.
+
+Inference workload: 1,000,000 options, 95% inside the training box and 5%
+outside it (a in [0.2, 0.4] or expiry 30-40 y); ensemble inference with the
+spread gate, options outside the box or above the gate's threshold re-priced
+by the FP64 tree in the same stream.
+
+Bars:
+- training: the last epoch's loss below 5% of the first;
+- gradients on one full 8,192-row batch: per tensor within 1e-2 (relative,
+  L2) of the BF16-emulating CPU reference (tests/surrogate_ref.py);
+- the gate's spread threshold is calibrated on a held-out sample of 250,000
+  options (its own seed) at half the error budget, 0.5e-3;
+- every one of the 1,000,000 prices within 1e-3 per 100 notional of the FP64
+  tree, and a stratified subset (network-priced, re-priced in-box and
+  out-of-box options) within 1e-3 of the oracle (FP64 seq restatement);
+- what the network gains: in-box max error of closed form + network below
+  that of the closed form alone.
+The fallback fraction and the error figures are printed.
+"""
+import numpy as np
+import pytest
+
+import surrogate_ref as R
+
+pytestmark = pytest.mark.gpu
+
+N_TRAIN = 2_000_000
+BATCH = 8192
+EPOCHS = 40
+N_MIX = 1_000_000
+BAR = 1e-3
+ORACLE_THREADS = 16
+
+
+@pytest.fixture(scope="module")
+def trained(pkg):
+    y = pkg.datasets.synthetic_yield()
+    S = pkg.surrogate
+    cfg = S.config(members=4, batch=BATCH, seed=5)
+    s, rec = S.fit(y, n_train=N_TRAIN, epochs=EPOCHS, seed=11, cfg=cfg, infer_capacity=N_MIX)
+    print(f"\ntrained {rec['members']} members on {rec['samples']} samples, batch {rec['batch']}: "
+          f"loss {rec['loss_first']:.3e} -> {rec['loss_last']:.3e} in {rec['train_s']:.1f} s", flush=True)
+    yield y, s, rec
+    s.close()
+
+
+def test_training_at_size_reduces_loss(trained):
+    _, s, rec = trained
+    assert rec["samples"] == N_TRAIN and rec["batch"] == BATCH and rec["members"] == 4
+    assert rec["steps"] == EPOCHS * (N_TRAIN // BATCH)
+    assert rec["loss_last"] < 0.05 * rec["loss_first"], rec
+
+
+def test_gradients_on_a_full_batch_match_cpu_reference(pkg, trained):
+    """One 8,192-row batch of the 2M-sample set through the GPU backward,
+    against the BF16-emulating CPU backward of the same member."""
+    y, _, _ = trained
+    S = pkg.surrogate
+    opts, dev, prices = S.training_set(11, 65536, y)
+    s = S.Surrogate(65536, 128, y.N, S.config(batch=BATCH, seed=23))
+    s.set_training_data(dev, prices)
+    s.train(2, 4)  # away from the initial weights
+    rows = (np.arange(BATCH) * 7 + 3) % 65536
+    _, _, g_gpu = s.eval_rows(1, rows)
+    x, t = s.get_inputs(rows)
+    P = s.get_params()[1].astype(np.float64)
+    y_cpu, cache = R.forward_bf16(P, x)
+    g_cpu = R.backward(cache, y_cpu.astype(np.float64), t.astype(np.float64))
+    for name, o in R.OFF.items():
+        size = int(np.prod(R.SHAPE[name]))
+        a, b = g_gpu[o:o + size].astype(np.float64), g_cpu[o:o + size]
+        assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-12, \
+            (name, np.linalg.norm(a - b), np.linalg.norm(b))
+    s.close()
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()
+
+
+def test_one_million_mix_within_bar_of_tree_and_oracle(pkg, oracle, trained):
+    y, s, _ = trained
+    S, capi = pkg.surrogate, pkg.capi
+    cal = S.calibrate_threshold(s, y, n=250_000, seed=77, target=BAR, margin=0.5)
+    gated = S.with_threshold(s, cal["threshold"], N_MIX, y.N)
+    mix = pkg.datasets.box_mix(5, N_MIX)
+    dev = capi.to_device(mix, y)
+    plan = S.fallback_plan(N_MIX, y.N)
+    plan.execute(dev)
+    plan.sync()
+    tree = dev["out"].to_host()
+    dev["out"].fill_zero()
+    st = gated.price(plan, dev)
+    got = dev["out"].to_host()
+    mean, spread, inbox = gated.predict(dev)
+    accepted = inbox & (spread <= gated.cfg.spread_threshold)
+    err = np.abs(got - tree)
+    frac = st["fallback"] / N_MIX
+    print(f"\n1M mix: threshold {cal['threshold']:.3e} (calibration accepted {cal['accepted_frac']:.3f}); "
+          f"fallback {st['fallback']} ({frac:.4f}), out of box {st['out_of_box']}; "
+          f"max abs err vs FP64 tree {err.max():.3e}, mean {err.mean():.3e}", flush=True)
+    assert np.all(np.isfinite(got))
+    assert st["out_of_box"] == int((~inbox).sum()) and st["out_of_box"] >= 0.049 * N_MIX
+    assert st["fallback"] == int((~accepted).sum())
+    assert err.max() <= BAR
+    # the re-priced options are the tree's own prices, bit for bit
+    assert np.array_equal(got[~accepted], tree[~accepted])
+    # the network prices most in-box options (a gate that rejects everything
+    # would pass the error bar trivially)
+    assert accepted.sum() >= 0.5 * N_MIX
+
+    # stratified oracle check: network-priced, re-priced in-box, out-of-box
+    rng = np.random.default_rng(3)
+    groups = [np.nonzero(accepted)[0], np.nonzero(inbox & ~accepted)[0], np.nonzero(~inbox)[0]]
+    sub = np.concatenate([rng.choice(g, min(len(g), k), replace=False) for g, k in zip(groups, (240, 60, 100))])
+    ref = oracle.price(mix.take(sub), y, threads=ORACLE_THREADS)
+    err_o = np.abs(got[sub] - ref)
+    print(f"stratified oracle subset ({len(sub)} options): max abs err {err_o.max():.3e}", flush=True)
+    assert err_o.max() <= BAR
+
+    # what the network gains over the closed form alone (output layer zeroed)
+    P0 = gated.get_params()
+    P0[:, pkg.surrogate.OFFSETS["W5"]:] = 0.0
+    s0 = S.Surrogate(0, N_MIX, y.N, S.config(members=4))
+    s0.set_params(P0)
+    base, _, _ = s0.predict(dev)
+    s0.close()
+    e_cf = np.abs(base[inbox] - tree[inbox])
+    e_net = np.abs(mean[inbox] - tree[inbox])
+    print(f"in-box: closed form max {e_cf.max():.3e} mean {e_cf.mean():.3e}; "
+          f"with network max {e_net.max():.3e} mean {e_net.mean():.3e}", flush=True)
+    assert e_net.max() < e_cf.max() and e_net.mean() < 0.2 * e_cf.mean()
+    gated.close()
+    plan.close()
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()

@@ -6,6 +6,10 @@
 * The reference's own seq binary built here from its sources (oracle/_ref):
   bit-identical output, all sets (skipped when the binary was not built).
 * Thesis known answers for Hull's book option (thesis/chapters/Sequential.tex:155-171).
+* The FP32 build's trees (price_f32shape): data/out32/<set>.out and the
+  reference's FP32 seq binary (oracle/_ref/Seq32), at the reference's own FP32
+  comparison (test/Compare.hpp:8-16), including step counts where the float
+  exercise step differs from the double one.
 """
 import numpy as np
 import pytest
@@ -64,3 +68,49 @@ def test_golden_sets_cover_edge_rows(pkg, golden_yield, oracle):
         t = c.n * c.dt
         p = 100.0 * np.exp(-oracle.yield_at_year(t, c.termUnit, golden_yield) * t)
         assert abs(gold[i] - p) < 1e-10
+
+
+FLT_EPS = float(np.finfo(np.float32).eps)
+
+
+def catch_ok(got, want):
+    """test/Compare.hpp:8-16: Approx(want).margin(1000 * FLT_EPSILON), whose
+    default relative epsilon is 100 * FLT_EPSILON of (1 + the larger magnitude)."""
+    d = np.abs(got - want)
+    return (d <= 1000 * FLT_EPS) | (d <= 100 * FLT_EPS * (1.0 + np.maximum(np.abs(got), np.abs(want))))
+
+
+@pytest.mark.parametrize("name", GOLDEN_SETS)
+def test_f32_shapes_match_reference_fp32_goldens(pkg, oracle, golden_yield, name):
+    opts = pkg.formats.read_options(GOLDEN / f"{name}.in")
+    gold32 = pkg.formats.read_results(GOLDEN / f"{name}.out32")
+    got = oracle.price_f32shape(opts, golden_yield)
+    ok = catch_ok(got, gold32)
+    assert ok.all(), (name, np.nonzero(~ok)[0][:5])
+
+
+def test_f32_shapes_match_reference_fp32_binary(pkg, oracle, golden_yield, tmp_path):
+    """Steps per year where the float exercise step (int)(t/dt) is one less
+    than the double one (15, 30, 60, 63, 75, 77 at t = 3): the FP32 reference
+    prices another tree there, and the restatement follows it."""
+    if not oracle.REF_SEQ32.exists():
+        pytest.skip("reference FP32 seq binary not built (needs the reference sources)")
+    steps = list(range(1, 101)) + [365]
+    opts = pkg.datasets.book(steps)
+    pkg.formats.write_options(tmp_path / "b.in", opts)
+    ref32 = np.array([float(t) for t in oracle.run_reference_seq32(tmp_path / "b.in", GOLDEN / "yield.in")])
+    got = oracle.price_f32shape(opts, golden_yield)
+    ok = catch_ok(got, ref32)
+    assert ok.all(), (np.nonzero(~ok)[0][:5], got[~ok][:5], ref32[~ok][:5])
+    def e_float(i):
+        c = oracle.constants_f32(opts, i)
+        return int(np.float32(c.t) / np.float32(c.dt))
+
+    def e_double(i):
+        c = oracle.constants(opts, i)
+        return int(c.t / c.dt)
+
+    shifted = [i for i in range(opts.N) if e_float(i) != e_double(i)]
+    assert len(shifted) >= 5  # the quirk is exercised
+    double = oracle.price(opts, golden_yield)
+    assert np.all(np.abs(double[shifted] - ref32[shifted]) > 1e-3)  # ... and visible
