@@ -26,7 +26,7 @@ EXPORTS = (
     "hwt_config_init", "hwt_price_f64", "hwt_plan_create", "hwt_plan_execute",
     "hwt_plan_sync", "hwt_plan_destroy", "hwt_price_f32", "hwt_price_constants_f64", "hwt_price_constants_f32",
     "hwt_batch_create", "hwt_batch_create_f32", "hwt_batch_price", "hwt_batch_device_memory", "hwt_batch_destroy",
-    "hwt_alphas_f64", "hwt_autotune", "hwt_max_width", "hwt_strerror",
+    "hwt_price_device_noalloc", "hwt_alphas_f64", "hwt_autotune", "hwt_max_width", "hwt_strerror",
     "hwt_last_error", "hwt_abi_version",
 )
 
@@ -126,6 +126,7 @@ def lib():
         L.hwt_batch_device_memory.restype = C.c_int64
         L.hwt_batch_destroy.argtypes = [P]
         L.hwt_batch_destroy.restype = None
+        L.hwt_price_device_noalloc.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, P]
         L.hwt_alphas_f64.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, P]
         L.hwt_autotune.argtypes = [C.POINTER(OptionsC), C.POINTER(YieldC), P, C.POINTER(ConfigC),
                                    C.POINTER(TuneReportC)]

@@ -1274,6 +1274,117 @@ int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, c
     return rc;
 }
 
+} // extern "C"
+
+// ------------------------------------------------- allocation-free pricing
+// hwt_price_device_noalloc keeps every buffer its kernels need beyond the
+// caller's inputs and output in this one static device array, which each
+// device holds from the module's load until the process ends.  Layout (bytes,
+// each part 256-aligned): OptConst[N], edge tables[N][36], bucket[N],
+// order[N + row pads], histogram, scatter cursors and offsets (1,024 each),
+// counters (32), and the wide-tree kernel's parked forward states (one
+// 64 x MAXK double slot per resident wave, at most NOALLOC_WIDE_SLOTS).
+constexpr int NOALLOC_MAX_OPTIONS = HWT_NOALLOC_MAX_OPTIONS;
+constexpr int NOALLOC_WIDE_SLOTS = 1024;
+constexpr size_t noalloc_part(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+constexpr size_t NOALLOC_O_CST = 0;
+constexpr size_t NOALLOC_O_EDGE = NOALLOC_O_CST + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * sizeof(OptConst));
+constexpr size_t NOALLOC_O_BUCKET = NOALLOC_O_EDGE + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * EDGE_TAB * 8);
+constexpr size_t NOALLOC_O_ORDER = NOALLOC_O_BUCKET + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * 4);
+constexpr size_t NOALLOC_O_HIST = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + ROWS * ROW_MAXK) * 4);
+constexpr size_t NOALLOC_O_CURSOR = NOALLOC_O_HIST + noalloc_part(NBUCKET * 4);
+constexpr size_t NOALLOC_O_OFFS = NOALLOC_O_CURSOR + noalloc_part(NBUCKET * 4);
+constexpr size_t NOALLOC_O_TALLY = NOALLOC_O_OFFS + noalloc_part(NBUCKET * 4);
+constexpr size_t NOALLOC_O_WIDE = NOALLOC_O_TALLY + noalloc_part(T_ALLOC * 4);
+constexpr size_t NOALLOC_BYTES = NOALLOC_O_WIDE + (size_t)NOALLOC_WIDE_SLOTS * WAVE * MAXK * 8;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+// The array's address on each device, taken once per device with that device
+// current (in host code `scratch` names a host-side stand-in, not device
+// memory), and the persistent grids, which need occupancy queries.
+struct NoallocDevice {
+    unsigned char *ptr = nullptr;
+    int grid_wide = 0, grid_narrow = 0;
+};
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+
+int noalloc_device(int dev, NoallocDevice &out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(HWT_EINVAL, "hwt_price_device_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoallocDevice &d = g_noalloc[dev];
+    if (!d.ptr) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(HWT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        int cus = 0, per_w = 0, per_n = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_w, (const void *)k_price_wide<double>, WAVE, 0) !=
+                hipSuccess ||
+            per_w < 1)
+            per_w = 2;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_n, (const void *)k_price_narrow<double>, WAVE, 0) !=
+                hipSuccess ||
+            per_n < 1)
+            per_n = 4;
+        d.grid_wide = std::min(per_w * cus, NOALLOC_WIDE_SLOTS);
+        d.grid_narrow = per_n * cus;
+        d.ptr = (unsigned char *)ptr;
+    }
+    out = d;
+    return HWT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int hwt_price_device_noalloc(const hwt_options_f64 *dopts, const hwt_yield_f64 *dyield, double *dout, void *stream)
+{
+    g_last_error.clear();
+    if (!valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
+        return fail(HWT_EINVAL, "hwt_price_device_noalloc: null or negative-size argument");
+    if (dopts->n > NOALLOC_MAX_OPTIONS)
+        return fail(HWT_ENOMEM, "hwt_price_device_noalloc: " + std::to_string(dopts->n) +
+                                    " options exceed the fixed scratch capacity of " +
+                                    std::to_string(NOALLOC_MAX_OPTIONS) + "; use a plan");
+    int dev = 0;
+    HWT_HIP(hipGetDevice(&dev));
+    NoallocDevice d;
+    const int rc = noalloc_device(dev, d);
+    if (rc) return rc;
+    // a plan-shaped view of the scratch: default settings (form 2, FP64,
+    // work-descending order, kernels in sequence), nothing allocated
+    hwt_plan v;
+    hwt_config_init(&v.cfg);
+    v.device = dev;
+    v.grid_wide = d.grid_wide;
+    v.grid_narrow = d.grid_narrow;
+    v.cst = (OptConst *)(d.ptr + NOALLOC_O_CST);
+    v.edge = (double *)(d.ptr + NOALLOC_O_EDGE);
+    v.bucket = (int32_t *)(d.ptr + NOALLOC_O_BUCKET);
+    v.order = (int32_t *)(d.ptr + NOALLOC_O_ORDER);
+    v.hist = (int32_t *)(d.ptr + NOALLOC_O_HIST);
+    v.cursor = (int32_t *)(d.ptr + NOALLOC_O_CURSOR);
+    v.offs = (int32_t *)(d.ptr + NOALLOC_O_OFFS);
+    v.tally = (int32_t *)(d.ptr + NOALLOC_O_TALLY);
+    v.scratch_wide = (double *)(d.ptr + NOALLOC_O_WIDE);
+    hipStream_t s = (hipStream_t)stream;
+    // the array keeps the last call's contents: counters start from zero by a
+    // fill kernel (no memset node, so a caller may capture the call in a graph)
+    (void)hipGetLastError();
+    k_reset<<<1, NBUCKET, 0, s>>>(v.tally, v.hist, v.cursor);
+    HWT_HIP(hipGetLastError());
+    return launch_pipeline(&v, dev_opts(*dopts), DevYield{dyield->n, dyield->rate, dyield->day}, dout, s, false);
+}
+
 int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, int64_t *offsets, double *alphas)
 {
     g_last_error.clear();

@@ -242,6 +242,22 @@ int hwt_batch_price(hwt_batch *batch, double *host_out, const hwt_config *cfg, h
 int64_t hwt_batch_device_memory(const hwt_batch *batch);
 void hwt_batch_destroy(hwt_batch *batch);
 
+/* Pricing with no allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  A variant of
+ * hwt_plan_execute with the default configuration (form 2, FP64): device
+ * arrays in, prices in input order into dev_out, enqueued on `stream` (a
+ * hipStream_t, NULL = the default stream) without waiting.  Its kernels'
+ * buffers live in one static device array per GPU (about 37 MB, held from the
+ * library's load until the process ends) sized for at most
+ * HWT_NOALLOC_MAX_OPTIONS options; larger batches return HWT_ENOMEM.  Nothing
+ * is allocated and no memset is issued.  Invalid or too-wide options come
+ * back NaN; no per-option status is reported.  Calls that overlap on one GPU
+ * (concurrent streams or host threads) share the array and race. */
+#define HWT_NOALLOC_MAX_OPTIONS 65536
+int hwt_price_device_noalloc(const hwt_options_f64 *dev_opts, const hwt_yield_f64 *dev_yield, double *dev_out,
+                             void *stream);
+
 /* Per-step alpha calibration of each option (alphas[0..n] of the reference's
  * forward pass, seq/Seq.hpp:56-106), host arrays.  alpha_offsets (n_opts+1
  * int64, host) is filled first when alphas == NULL so the caller can size the
