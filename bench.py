@@ -35,41 +35,135 @@ FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_NODE_UPDATE = 5
 
 
-def cpu_baseline(pkg, opts, y):
-    """The reference's seq binary (REAL=64) on the same 60,000-option batch:
-    once with one OpenMP thread per host core (one option per core at a time)
-    and once single-threaded (about 20 s of CPU work on a current server
-    core).  Without the reference binary the oracle's C restatement of seq is
-    timed instead ("port")."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pkg, opts, y, c1):
+    """The reference's seq binary (REAL=64) on the same 60,000-option batch,
+    timed by its own -r mode (best of the runs, seq::computeOptions only, no
+    file I/O): with one OpenMP thread per host core (one option per core at a
+    time, best of three) and single-threaded (one run, about 20 s of CPU work
+    on a current server core); and single-threaded on the 1,000-put workload
+    (best of three).  Without the reference binary the oracle's C restatement
+    of seq is timed instead ("port")."""
     import oracle  # the reference binary lives beside the oracle (oracle/_ref)
     seq = oracle.REF_SEQ
-    kind = "reference"
+    kind = "reference" if seq.exists() else "port"
     ncores = os.cpu_count() or 1
     with tempfile.TemporaryDirectory() as td:
         td = Path(td)
         pkg.formats.write_yield(td / "yield.in", y)
 
-        def run(sample, threads):
+        def run(sample, threads, runs):
             pkg.formats.write_options(td / "opts.in", sample)
             env = {**os.environ, "OMP_NUM_THREADS": str(threads)}
-            t0 = time.perf_counter()
             if seq.exists():
-                subprocess.run([str(seq), "-o", str(td / "opts.in"), "-y", str(td / "yield.in")],
-                               check=True, capture_output=True, env=env)
-            else:
+                out = subprocess.run([str(seq), "-o", str(td / "opts.in"), "-y", str(td / "yield.in"),
+                                      "-r", str(runs)], check=True, capture_output=True, text=True, env=env).stdout
+                return int(out.strip().split(",")[4]) * 1e-6  # "-,-,-,-,<best us>,-"
+            best = None
+            for _ in range(runs):
+                t0 = time.perf_counter()
                 oracle.price(sample, y, threads=threads)
-            return time.perf_counter() - t0
+                t = time.perf_counter() - t0
+                best = t if best is None else min(best, t)
+            return best
 
-        if not seq.exists():
-            kind = "port"
-        t_multi = run(opts, ncores)
-        t_single = run(opts, 1)
+        t_multi = run(opts, ncores, 3)
+        t_single = run(opts, 1, 1)
+        t_c1 = run(c1, 1, 3)
     what = "reference seq (REAL=64, OpenMP)" if kind == "reference" else "oracle restatement of seq"
     return {"value": opts.N / t_multi, "unit": "options/s", "cores": ncores, "kind": kind,
-            "sample": f"the whole 60k skewed batch, {what} with {ncores} threads, {t_multi:.2f} s "
-                      f"(includes reading the text input files)",
+            "cpu_model": _cpu_model(),
+            "sample": f"the whole 60k skewed batch, {what} with {ncores} threads, best of 3 runs: {t_multi:.3f} s "
+                      f"(seq::computeOptions only, timed by the binary's -r mode)",
             "single_thread_value": opts.N / t_single,
-            "single_thread_sample": f"the whole 60k skewed batch, 1 thread, {t_single:.2f} s"}
+            "single_thread_sample": f"the whole 60k skewed batch, 1 thread, one run: {t_single:.2f} s",
+            "config1_single_thread_value": c1.N / t_c1,
+            "config1_sample": f"1,000 puts (width 47, 120 steps), 1 thread, best of 3: {t_c1 * 1e3:.1f} ms"}
+
+
+def reference_sets_leg(pkg, capi, args):
+    """The reference's seven 65,536-option benchmark distributions (drawn by
+    the seeded generator; the reference's 15-knot curve), each priced whole
+    by the FP64 tree: graph replays timed with HIP events, the autotuned
+    configuration.  The reference's own figures for these sets are on a
+    GTX 780 Ti (CUDA-multi FP64, 4_SKEWED: 1.147 s)."""
+    import ctypes as C
+    dm = pkg.devmem
+    yref = pkg.formats.read_yield(ROOT / "tests" / "golden" / "yield.in")
+    stream = C.c_void_p()
+    dm._ck(dm.hip().hipStreamCreate(C.byref(stream)))
+    out = {}
+    for name in pkg.datasets.REFERENCE_SETS:
+        opts = pkg.datasets.reference_set(name, 65536, seed=5)
+        dev = capi.to_device(opts, yref)
+        cfg, _ = capi.autotune(dev)
+        plan = capi.Plan(opts.N, yref.N, cfg)
+        for _ in range(3):
+            plan.execute(dev, stream=stream.value)
+        st = plan.sync()
+        e0, e1 = dm.Event(), dm.Event()
+        reps = 10
+        e0.record(stream)
+        for _ in range(reps):
+            plan.execute(dev, stream=stream.value)
+        e1.record(stream)
+        e1.synchronize()
+        plan.sync()
+        plan.close()
+        ms = e0.elapsed_ms(e1) / reps
+        out[name] = {"ms": ms, "options_per_s": opts.N / (ms * 1e-3), "node_updates": st["node_updates"],
+                     "seq_node_updates": st["seq_node_updates"], "tuned_flags": cfg.flags}
+        for k in list(dev):
+            if hasattr(dev[k], "free"):
+                dev[k].free()
+    dm._ck(dm.hip().hipStreamDestroy(stream))
+    return out
+
+
+def tiny_trees_leg(pkg, capi, y, n: int = 1_000_000):
+    """The flattened form against the row form on a batch of tiny trees
+    (widths 3-21, uniform): the same options priced with trees packed 64 / w
+    to a wavefront one node per lane (default), four per wavefront in 16-lane
+    rows (HWT_FLAG_NO_FLAT), and one per wavefront (also HWT_FLAG_NO_ROWS).
+    Graph replays timed with HIP events."""
+    import ctypes as C
+    dm = pkg.devmem
+    opts = pkg.datasets.tiny(7, n)
+    dev = capi.to_device(opts, y)
+    stream = C.c_void_p()
+    dm._ck(dm.hip().hipStreamCreate(C.byref(stream)))
+    out = {"options": n}
+    for name, flags in (("flat", 0), ("rows", capi.FLAG_NO_FLAT),
+                        ("one_per_wave", capi.FLAG_NO_FLAT | capi.FLAG_NO_ROWS)):
+        plan = capi.Plan(n, y.N, capi.config(flags=flags))
+        for _ in range(3):
+            plan.execute(dev, stream=stream.value)
+        st = plan.sync()
+        e0, e1 = dm.Event(), dm.Event()
+        e0.record(stream)
+        for _ in range(10):
+            plan.execute(dev, stream=stream.value)
+        e1.record(stream)
+        e1.synchronize()
+        plan.sync()
+        plan.close()
+        ms = e0.elapsed_ms(e1) / 10
+        out[name] = {"ms": ms, "options_per_s": n / (ms * 1e-3), "tasks": st["tasks"]}
+    out["flat_speedup_over_rows"] = out["rows"]["ms"] / out["flat"]["ms"]
+    dm._ck(dm.hip().hipStreamDestroy(stream))
+    for k in list(dev):
+        if hasattr(dev[k], "free"):
+            dev[k].free()
+    return out
 
 
 def surrogate_leg(pkg, y, args):
@@ -87,7 +181,10 @@ def surrogate_leg(pkg, y, args):
     ev = S.evaluate(gated, y, n=1_000_000, seed=5, reps=max(1, min(args.steps, 10)))
     gated.close()
     s.close()
-    flop = ev["options"] * 4 * 2 * (8 * 256 + 3 * 256 * 256 + 256)
+    # the inference kernel evaluates the in-box options only (the box split
+    # sends the others straight to the tree)
+    in_box = ev["options"] - ev["out_of_box"]
+    flop = in_box * 4 * 2 * (8 * 256 + 3 * 256 * 256 + 256)
     achieved = flop / (ev["inference_ms"] * 1e-3) / 1e12
     return {"workload": "1,000,000 options: 95% inside the training box, 5% outside (a in [0.2,0.4] "
                         "or expiry 30-40y); ensemble BF16 MLP inference + FP64 tree fallback",
@@ -95,8 +192,9 @@ def surrogate_leg(pkg, y, args):
             "roofline": {"bound": "mfma", "kernel": "k_infer", "achieved": achieved, "peak": 2500.0,
                          "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
                          "flop_per_launch": flop,
-                         "flop_model": "4 members x 2 x (8*256 + 3*256*256 + 256) FLOP per option, every row "
-                                       "of every tile; BF16 MFMA dense peak"}}
+                         "flop_model": "4 members x 2 x (8*256 + 3*256*256 + 256) FLOP per in-box option "
+                                       "(the rows the kernel evaluates); time = box split + inference; "
+                                       "BF16 MFMA dense peak"}}
 
 
 def main():
@@ -106,6 +204,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-reference-sets", action="store_true",
+                    help="skip the seven 65,536-option reference distributions in the detail")
     ap.add_argument("--surrogate-epochs", type=int, default=120,
                     help="training epochs of the surrogate ensemble over its 2M tree-priced samples")
     ap.add_argument("--profile", action="store_true",
@@ -290,12 +390,19 @@ def main():
             },
         },
     }
+    if not args.no_reference_sets:
+        line["detail"]["reference_sets_65536"] = reference_sets_leg(pkg, capi, args)
+        line["detail"]["tiny_trees_1m"] = tiny_trees_leg(pkg, capi, y)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg(pkg, y, args)
     if not args.no_cpu_baseline:
         sys.path.insert(0, str(ROOT / "oracle"))
-        line["cpu_baseline"] = cpu_baseline(pkg, opts, y)
-        line["cpu_baseline"]["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+        line["cpu_baseline"] = cpu_baseline(pkg, opts, y, pkg.datasets.uniform_1k())
+        cb = line["cpu_baseline"]
+        cb["gpu_over_cpu"] = line["value"] / cb["value"]
+        cb["gpu_over_cpu_single_thread"] = line["value"] / cb["single_thread_value"]
+        mp = line["detail"]["mixed_precision"]
+        mp["options_per_s_over_config1_seq"] = mp["options_per_s"] / cb["config1_single_thread_value"]
     print(json.dumps(line), flush=True)
 
 

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "../../include/hwtree.h"
+#include "hwt_flat.h"
 #include "hwt_rows.h"
 #include "hwt_wave.h"
 
@@ -37,16 +38,24 @@ struct DevYield {
 //   wide  (w > 64*KSMALL)           k_price_wide, one option per wave, LPT queue
 //   mid   (ROW_MAX_WIDTH < w <= 64*KSMALL)  k_price_narrow, one option per wave
 //   row   (w <= ROW_MAX_WIDTH)      k_price_narrow, four options per wave
-// Bucket ranges: wide [B_WIDE, 1024) and mid [B_MID, B_WIDE), log-spaced by
-// the sort key (work unless a width/height order is asked for); row
-// [0, B_MID), by (nodes per lane, height) so the four rows of a wave share
-// their step count.  The scan walks each class from its largest key down.
-constexpr int NBUCKET = 1024;
-constexpr int B_MID = 640, B_WIDE = 768;
+//   flat  (w <= FLAT_MAX_WIDTH)     k_price_narrow, 64 / w options per wave, one
+//                                   node per lane (hwt_flat.h)
+// Bucket ranges: wide [B_WIDE, NBUCKET) and mid [B_MID, B_WIDE), log-spaced
+// by the sort key (work unless a width/height order is asked for); row
+// [B_ROW, B_MID), by (nodes per lane, height) so the four rows of a wave share
+// their step count; flat [0, B_ROW), by (width, height).  The scan walks each
+// class from its largest key down.
+constexpr int NBUCKET = 2048;
+constexpr int B_ROW = 640, B_MID = 1280, B_WIDE = 1536;
+constexpr int ROW_HBUCKETS = 160;   // height buckets per row part (nodes per lane 1..4)
+constexpr int FLAT_HBUCKETS = 64;   // height buckets per flat part (one width each)
+constexpr int MAX_PADS = 128;       // idle task slots padding row and flat parts, at most
 constexpr int KSMALL = 2;  // trees up to 64*KSMALL nodes wide go to the narrow-tree kernel
 #ifndef WIDE_WINDOW_STEP
 #define WIDE_WINDOW_STEP 4  // nodes per lane a wide tree's window grows or shrinks by
 #endif
+// k_setup's class switches
+constexpr int CLASS_ROWS = 1, CLASS_FLAT = 2;
 enum Tally {
     T_QUEUE = 0,    // next task of the wide-tree kernel (tasks [0, T_NLARGE))
     T_QUEUE_S = 1,  // (unused: the narrow-tree kernel strides statically)
@@ -70,8 +79,15 @@ enum Tally {
     A_UNSUP = 23,
     T_NREDO = 24,   // mixed mode: options handed to the FP64 re-pricing (host-visible)
     T_QUEUE_X = 25, // next task of the FP64 re-pricing kernel
-    T_NCOUNT_ALL = 26,  // [0, T_NCOUNT_ALL) is copied to the host after an execution
-    T_ALLOC = 32
+    T_NROW = 26,    // end of the row-class tasks (start of the flat class)
+    T_QUEUE_F = 27, // next chunk of flat groups (fused kernel)
+    T_NFLATG = 28,  // flat groups in all
+    T_NCOUNT_ALL = 29,  // [0, T_NCOUNT_ALL) is copied to the host after an execution
+    // first task and first group of each flat part (one width each, widest
+    // first, as the scan lays them out), then the end of the last part
+    T_FLAT_START = 32,  // FLAT_PARTS + 1 entries
+    T_FLAT_GROUP = 48,  // FLAT_PARTS + 1 entries
+    T_ALLOC = 64
 };
 
 // Pricing kernels (hwt_pricers.h).  All are persistent: one single-wave

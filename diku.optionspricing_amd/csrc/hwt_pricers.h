@@ -269,6 +269,64 @@ __device__ __forceinline__ void price_row_groups(const OptConst *__restrict__ cs
     }
 }
 
+// Flat-class groups, one per wave at a time: group G of the flat class is
+// part j's group G - T_FLAT_GROUP[j] (parts hold one width each, widest
+// first), 64 / w consecutive tasks from T_FLAT_START[j].  A part's padding
+// slots come last, so a group's first task is always a real one; lanes of an
+// idle tree price nothing.
+//   DYNAMIC: groups are claimed ROW_CHUNK at a time from tally[T_QUEUE_F]
+//   (the fused kernel); otherwise wave b takes groups b, b + grid, ...
+template <class T, bool DYNAMIC>
+__device__ __forceinline__ void price_flat_groups(const OptConst *__restrict__ cst, const double *__restrict__ edge,
+                                                  const int32_t *__restrict__ order, double *__restrict__ out,
+                                                  int32_t *__restrict__ tally, int32_t *__restrict__ redo)
+{
+    __shared__ double flatbuf[2 * WAVE + 4 * flat_per_wave(3)];
+    const int lane = threadIdx.x;
+    lds_f64 *ybuf = (lds_f64 *)flatbuf, *bbuf = ybuf + WAVE, *sbuf = bbuf + WAVE;
+    const int ngroups = uni(tally[T_NFLATG]);
+    int G, gend;
+    auto claim = [&]() {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&tally[T_QUEUE_F], 1);
+        c = uni(__shfl(c, 0, WAVE));
+        G = c * ROW_CHUNK;
+        gend = min(G + ROW_CHUNK, ngroups);
+    };
+    if (DYNAMIC) claim();
+    else G = uni((int)blockIdx.x);
+    while (G < ngroups) {
+        int j = 0;
+        while (j + 1 < FLAT_PARTS && uni(tally[T_FLAT_GROUP + j + 1]) <= G) ++j;
+        const int w = flat_width(FLAT_PARTS - 1 - j), g = flat_per_wave(w);
+        const int start = uni(tally[T_FLAT_START + j]) + (G - uni(tally[T_FLAT_GROUP + j])) * g;
+        const int q = lane / w, k = lane - q * w;
+        const int mine = q < g ? order[start + q] : -1;
+        const bool active = mine >= 0;
+        const int o = active ? mine : order[start];
+        const OptConst c = cst[o];
+        RowOpt r;
+        r.t = TreeShape{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
+        r.n = c.n;
+        r.e = c.e;
+        r.type = c.type;
+        r.X = c.X;
+        r.Pe = c.Pe;
+        r.Pn = c.Pn;
+        r.edge = edge + (size_t)o * EDGE_TAB;
+        const double p = price_flat<T>(r, k, active, g, w, ybuf, bbuf, sbuf);
+        if (lane < g) {
+            const int oq = order[start + lane];
+            if (oq >= 0) store_price<T>(p, oq, out, tally, redo);
+        }
+        if (DYNAMIC) {
+            if (++G >= gend) claim();
+        } else {
+            G += uni((int)gridDim.x);
+        }
+    }
+}
+
 // Four waves per SIMD (at most 128 VGPRs): on a 1M batch of mostly tall
 // narrow trees the kernel runs 5% faster than at the three its 149 VGPRs
 // would otherwise allow.
@@ -277,9 +335,10 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
                                                        const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                                                        double *__restrict__ out, int32_t *__restrict__ redo)
 {
-    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
+    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), nrow = uni(tally[T_NROW]);
     price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr, redo);
-    price_row_groups<T, false>(cst, edge, order, nmid, ntask, out, tally, redo);
+    price_row_groups<T, false>(cst, edge, order, nmid, nrow, out, tally, redo);
+    price_flat_groups<T, false>(cst, edge, order, out, tally, redo);
 }
 
 // Every class in one persistent kernel: the wide trees from the LPT queue,
@@ -291,10 +350,11 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
     int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch, int32_t *__restrict__ redo)
 {
-    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), ntask = uni(tally[T_NTASK]);
+    const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), nrow = uni(tally[T_NROW]);
     price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch, redo);
     price_tasks<T, KSMALL, true, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr, redo);
-    price_row_groups<T, true>(cst, edge, order, nmid, ntask, out, tally, redo);
+    price_row_groups<T, true>(cst, edge, order, nmid, nrow, out, tally, redo);
+    price_flat_groups<T, true>(cst, edge, order, out, tally, redo);
 }
 
 } // namespace hwt

@@ -634,6 +634,7 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
 struct PlainSum {
     double s = 0.0;
     __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double total() const { return s; }
     __device__ __forceinline__ double wave_total() const { return wave_sum(s); }
     __device__ __forceinline__ double row_total() const { return row_sum(s); }
 };
@@ -646,6 +647,7 @@ struct KahanSum {
         c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
         s = t;
     }
+    __device__ __forceinline__ double total() const { return s + c; }
     __device__ __forceinline__ double wave_total() const { return wave_sum(s) + wave_sum(c); }
     __device__ __forceinline__ double row_total() const { return row_sum(s) + row_sum(c); }
 };

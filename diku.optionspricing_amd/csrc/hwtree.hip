@@ -61,16 +61,33 @@ __device__ __forceinline__ int log_bucket(int64_t key, int per_octave, int nb)
 __device__ __forceinline__ int row_bucket(int w, int n)
 {
     const int K = (w + ROW - 1) / ROW;
-    const int h = n < 96 ? n : min(159, 96 + (int)(16.0 * log2(n / 96.0)));
-    return (K - 1) * 160 + h;
+    const int h = n < 96 ? n : min(ROW_HBUCKETS - 1, 96 + (int)(16.0 * log2(n / 96.0)));
+    return B_ROW + (K - 1) * ROW_HBUCKETS + h;
 }
+
+// Flat class: 64 height buckets per width: exact heights below 32, then 8
+// buckets per octave.
+__device__ __forceinline__ int flat_bucket(int w, int n)
+{
+    const int h = n < 32 ? n : min(FLAT_HBUCKETS - 1, 32 + (int)(8.0 * log2(n / 32.0)));
+    return flat_part(w) * FLAT_HBUCKETS + h;
+}
+
+// The part of the row or flat class bucket b belongs to: 0..ROW_MAXK-1 the
+// row parts (nodes per lane), then the flat parts (one per width); and the
+// number of tasks a wave takes from that part at once.
+__device__ __forceinline__ int part_of(int b) { return b >= B_ROW ? (b - B_ROW) / ROW_HBUCKETS : ROW_MAXK + b / FLAT_HBUCKETS; }
+__device__ __forceinline__ int part_group(int part)
+{
+    return part < ROW_MAXK ? ROWS : flat_per_wave(flat_width(part - ROW_MAXK));
+}
+constexpr int NPARTS = ROW_MAXK + FLAT_PARTS;
 
 // Every counter and tally to zero: the empty batch's whole pipeline.
 __global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
 {
-    const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i < T_ALLOC) tally[i] = 0;
-    if (i < NBUCKET) {
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < NBUCKET; i += blockDim.x * gridDim.x) {
+        if (i < T_ALLOC) tally[i] = 0;
         hist[i] = 0;
         cursor[i] = 0;
     }
@@ -87,7 +104,7 @@ static_assert(NBUCKET % SETUP_BLOCK == 0, "scan positions per thread");
 
 // Bucket at scan position p: the wide class, then the mid class, each largest
 // key first unless an ascending order was asked for, then the row class by
-// decreasing (K, height).
+// decreasing (K, height), then the flat class by decreasing (width, height).
 __device__ __forceinline__ int scan_bucket(int p, int asc)
 {
     constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID;
@@ -99,18 +116,19 @@ __device__ __forceinline__ int scan_bucket(int p, int asc)
 // One block of SETUP_BLOCK threads, after k_setup: exclusive offsets of the
 // buckets in scan order, the class boundaries and counts into the
 // host-visible tally, and every counter cleared for the next execution
-// (histogram, scatter cursors, queue heads, accumulators).  Each K part of the
-// row class is padded to a multiple of four tasks with idle slots (order -1),
-// so every group of four row tasks shares one K and an option's arithmetic
-// never depends on which options share its wave (not for the lane form,
-// which has no row groups).
+// (histogram, scatter cursors, queue heads, accumulators).  Each part of the
+// row class (one K) is padded to a multiple of four tasks and each part of the
+// flat class (one width w) to a multiple of 64 / w, with idle slots (order
+// -1), so every group a wave takes shares one K or one width and an option's
+// arithmetic never depends on which options share its wave.  The flat parts'
+// first tasks and first group numbers go to the tally for the flat kernel.
 __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist, int32_t *__restrict__ offs,
                                                       int32_t *__restrict__ cursor, int32_t *__restrict__ order,
-                                                      int32_t *__restrict__ tally, int32_t asc, int32_t pad_rows)
+                                                      int32_t *__restrict__ tally, int32_t asc, int32_t classes)
 {
     const int tid = threadIdx.x;
-    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NKB = B_MID / ROW_MAXK;
-    __shared__ int32_t ktot[ROW_MAXK];
+    constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NR = B_MID - B_ROW;
+    __shared__ int32_t ptot[NPARTS];
     __shared__ int32_t part[SETUP_BLOCK];
     __shared__ int32_t npad;
     int idx[SCAN_PER], cnt[SCAN_PER];
@@ -119,21 +137,24 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
         idx[q] = scan_bucket(tid * SCAN_PER + q, asc);
         cnt[q] = hist[idx[q]];
     }
-    if (tid < ROW_MAXK) ktot[tid] = 0;
+    if (tid < NPARTS) ptot[tid] = 0;
     if (tid == 0) npad = 0;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
         hist[idx[q]] = 0;
         cursor[idx[q]] = 0;
-        if (idx[q] < B_MID && cnt[q]) atomicAdd(&ktot[idx[q] / NKB], cnt[q]);
+        if (idx[q] < B_MID && cnt[q]) atomicAdd(&ptot[part_of(idx[q])], cnt[q]);
     }
     __syncthreads();
-    // the lowest bucket of each K part comes last in the scan and takes the pad
+    // the lowest bucket of each part comes last in the scan and takes the pad
     int v[SCAN_PER], pad[SCAN_PER], sum = 0, pads = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
-        pad[q] = (pad_rows && idx[q] < B_MID && idx[q] % NKB == 0) ? (ROWS - ktot[idx[q] / NKB] % ROWS) % ROWS : 0;
+        const int b = idx[q];
+        const bool lowest = b < B_MID && (b >= B_ROW ? (b - B_ROW) % ROW_HBUCKETS == 0 : b % FLAT_HBUCKETS == 0);
+        const int g = b < B_MID ? part_group(part_of(b)) : 1;
+        pad[q] = (classes && lowest) ? (g - ptot[part_of(b)] % g) % g : 0;
         v[q] = cnt[q] + pad[q];
         sum += v[q];
         pads += pad[q];
@@ -156,13 +177,30 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
         for (int j = 0; j < pad[q]; ++j) order[run - pad[q] + j] = -1;
         if (p == NW - 1) tally[T_NLARGE] = run;
         if (p == NW + NM - 1) tally[T_NMID] = run;
+        if (p == NW + NM + NR - 1) tally[T_NROW] = run;
         if (p == NBUCKET - 1) tally[T_NTASK] = run;
     }
+    __syncthreads();  // orders the boundary writes above before thread 0 reads them
     if (tid == 0) {
+        // flat parts in task order, widest first: first task and first group
+        int start = tally[T_NROW], group = 0;
+        for (int j = 0; j < FLAT_PARTS; ++j) {
+            const int fp = FLAT_PARTS - 1 - j;
+            const int g = flat_per_wave(flat_width(fp));
+            const int size = ptot[ROW_MAXK + fp] + (classes ? (g - ptot[ROW_MAXK + fp] % g) % g : 0);
+            tally[T_FLAT_START + j] = start;
+            tally[T_FLAT_GROUP + j] = group;
+            start += size;
+            group += size / g;
+        }
+        tally[T_FLAT_START + FLAT_PARTS] = start;
+        tally[T_FLAT_GROUP + FLAT_PARTS] = group;
+        tally[T_NFLATG] = group;
         tally[T_NPAD] = npad;
         tally[T_QUEUE] = 0;
         tally[T_QUEUE_S] = 0;
         tally[T_QUEUE_R] = 0;
+        tally[T_QUEUE_F] = 0;
         int64_t *t64 = reinterpret_cast<int64_t *>(tally);
         t64[T_WORK / 2] = t64[A_WORK / 2];
         t64[T_SEQWORK / 2] = t64[A_SEQWORK / 2];
@@ -180,7 +218,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
 }
 
 __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
-                                                       double unit_dmin, int32_t rows,
+                                                       double unit_dmin, int32_t classes,
                                                        OptConst *__restrict__ cst, double *__restrict__ edge,
                                                        int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
                                                        int32_t *__restrict__ tally, double *__restrict__ out)
@@ -291,7 +329,8 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 }
                 const bool wide = (c.w + WAVE - 1) / WAVE > KSMALL;
                 if (wide) b = B_WIDE + log_bucket(key, 8, NBUCKET - B_WIDE);
-                else if (c.w > ROW_MAX_WIDTH || !rows) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
+                else if ((classes & CLASS_FLAT) && c.w <= FLAT_MAX_WIDTH) b = flat_bucket(c.w, c.n);
+                else if (c.w > ROW_MAX_WIDTH || !(classes & CLASS_ROWS)) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
                 else b = row_bucket(c.w, c.n);
                 atomicAdd(&lh[b], 1);
                 if (wide) atomicAdd(&lsum[2], (unsigned long long)c.work);
@@ -496,17 +535,20 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
     // an empty batch launches no k_setup and no k_scan: the tallies it
     // reports (all zero) come from a clearing kernel instead
-    if (n == 0) k_reset<<<1, NBUCKET, 0, s>>>(p->tally, p->hist, p->cursor);
+    if (n == 0) k_reset<<<1, 1024, 0, s>>>(p->tally, p->hist, p->cursor);
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
         const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
-        const bool rows = p->cfg.form != HWT_FORM_LANE && !(p->cfg.flags & HWT_FLAG_NO_ROWS);
-        // the lane form prices tasks one per lane: no row groups, no pads
+        const int classes = p->cfg.form == HWT_FORM_LANE
+                                ? 0
+                                : ((p->cfg.flags & HWT_FLAG_NO_ROWS) ? 0 : CLASS_ROWS) |
+                                      ((p->cfg.flags & HWT_FLAG_NO_FLAT) ? 0 : CLASS_FLAT);
+        // the lane form prices tasks one per lane: no row or flat groups, no pads
         // form 1 prices trees up to its column length; wider ones are unsupported
         const int max_width = p->cfg.form == HWT_FORM_LANE ? p->lane_width : WAVE_MAX_WIDTH;
-        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, max_width, unit_dmin, rows, p->cst, p->edge,
+        k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, max_width, unit_dmin, classes, p->cst, p->edge,
                                                p->bucket, p->hist, p->tally, out);
-        k_scan<<<1, SETUP_BLOCK, 0, s>>>(p->hist, p->offs, p->cursor, p->order, p->tally, asc, rows);
+        k_scan<<<1, SETUP_BLOCK, 0, s>>>(p->hist, p->offs, p->cursor, p->order, p->tally, asc, classes);
         k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
@@ -686,7 +728,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_cst = carve(cap * sizeof(OptConst));
     const size_t o_edge = carve(cap * EDGE_TAB * sizeof(double));
     const size_t o_bucket = carve(cap * sizeof(int32_t));
-    const size_t o_order = carve((cap + ROWS * ROW_MAXK) * sizeof(int32_t));  // + row-class pads
+    const size_t o_order = carve((cap + MAX_PADS) * sizeof(int32_t));  // + row and flat pads
     const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
     const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
     const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
@@ -1291,7 +1333,7 @@ constexpr size_t NOALLOC_O_CST = 0;
 constexpr size_t NOALLOC_O_EDGE = NOALLOC_O_CST + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * sizeof(OptConst));
 constexpr size_t NOALLOC_O_BUCKET = NOALLOC_O_EDGE + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * EDGE_TAB * 8);
 constexpr size_t NOALLOC_O_ORDER = NOALLOC_O_BUCKET + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * 4);
-constexpr size_t NOALLOC_O_HIST = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + ROWS * ROW_MAXK) * 4);
+constexpr size_t NOALLOC_O_HIST = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + MAX_PADS) * 4);
 constexpr size_t NOALLOC_O_CURSOR = NOALLOC_O_HIST + noalloc_part(NBUCKET * 4);
 constexpr size_t NOALLOC_O_OFFS = NOALLOC_O_CURSOR + noalloc_part(NBUCKET * 4);
 constexpr size_t NOALLOC_O_TALLY = NOALLOC_O_OFFS + noalloc_part(NBUCKET * 4);
@@ -1380,7 +1422,7 @@ int hwt_price_device_noalloc(const hwt_options_f64 *dopts, const hwt_yield_f64 *
     // the array keeps the last call's contents: counters start from zero by a
     // fill kernel (no memset node, so a caller may capture the call in a graph)
     (void)hipGetLastError();
-    k_reset<<<1, NBUCKET, 0, s>>>(v.tally, v.hist, v.cursor);
+    k_reset<<<1, 1024, 0, s>>>(v.tally, v.hist, v.cursor);
     HWT_HIP(hipGetLastError());
     return launch_pipeline(&v, dev_opts(*dopts), DevYield{dyield->n, dyield->rate, dyield->day}, dout, s, false);
 }
@@ -1473,7 +1515,7 @@ void tune_store_file(const std::string &key, const hwt_config &c)
     const char *path = getenv("HWT_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
-    const int kept = c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED);
+    const int kept = c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT);
     f << key << ' ' << c.form << ' ' << c.sort << ' ' << kept << ' '
       << c.wide_per_cu << ' ' << c.narrow_per_cu << '\n';
 }
@@ -1555,8 +1597,8 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
             hwt_config c = base;
             c.form = it->second.form;
             c.sort = it->second.sort;
-            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED)) |
-                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED));
+            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT)) |
+                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT));
             c.wide_per_cu = it->second.wide_per_cu;
             c.narrow_per_cu = it->second.narrow_per_cu;
             *cfg = c;
@@ -1591,7 +1633,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         c.sort = so;
         if ((rc = consider(c))) return rc;
     }
-    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_CONCURRENT, HWT_FLAG_FUSED}) {
+    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_NO_FLAT, HWT_FLAG_CONCURRENT, HWT_FLAG_FUSED}) {
         hwt_config c = best;
         c.flags ^= bit;
         if ((rc = consider(c))) return rc;
@@ -1645,3 +1687,9 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
     return HWT_OK;
 }
 
+
+static_assert(ROW_MAXK * (ROWS - 1) + [] {
+    int pads = 0;
+    for (int p = 0; p < FLAT_PARTS; ++p) pads += flat_per_wave(flat_width(p)) - 1;
+    return pads;
+}() <= MAX_PADS, "task-list room for the row and flat padding");

@@ -15,10 +15,22 @@ Workloads:
   box            surrogate training box: strike [60,100], option 1-30y, bond =
                  option + [1,10]y, 12 or 36 steps/yr, a [0.01,0.2], sigma [0.005,0.02]
   box_mix        95% inside the box, 5% outside (a in [0.2,0.4] or option 30-40y)
+  tiny           trees 3-21 nodes wide (a from 3.0 down to 0.09 at 12 steps/yr),
+                 bond 1-30y, option U[0.2,0.8] x bond, strike U[60,100], call/put
 The yield curve for all of them has 120 monthly knots, zero rate rising
 linearly from 1% to 4%.
+
+reference_set(kind, n) draws the reference's seven benchmark distributions
+(data-generator/RandomDataGenerator.cpp:114-257): 0_UNIFORM, 1_RAND,
+2_RANDCONSTHEIGHT, 3_RANDCONSTWIDTH, 4_SKEWED, 5_SKEWEDCONSTHEIGHT and
+6_SKEWEDCONSTWIDTH, with the reference's widths, heights, skew fraction and
+fixed fields, from this module's own seeded generator (the reference's C
+rand() stream is not reproduced, so the rows differ; the distributions do
+not).  They price on the reference's 15-knot curve (tests/golden/yield.in).
 """
 from __future__ import annotations
+
+import functools
 
 import numpy as np
 
@@ -93,6 +105,17 @@ def box_mix(seed: int, n: int, outside: float = 0.05) -> Options:
     return inside
 
 
+def tiny(seed: int, n: int) -> Options:
+    """Options on trees 3 to 21 nodes wide (widths uniform over the odd
+    values), the flattened form's class."""
+    rng = np.random.default_rng(seed)
+    widths = rng.choice(np.arange(3, 22, 2), n)
+    a = np.array([3.0 if w == 3 else reference_rate_for_width(int(w)) for w in widths])
+    mat = rng.uniform(1.0, 30.0, n)
+    return _opts(rng.uniform(60, 100, n), mat, rng.uniform(0.2, 0.8, n) * mat, np.full(n, 12, np.uint16), a,
+                 rng.uniform(0.005, 0.02, n), rng.integers(0, 2, n).astype(np.int8))
+
+
 def book(steps=(1, 2, 3, 4, 5)) -> Options:
     """Hull's book example: 3-year put on a 9-year zero-coupon bond, strike 63,
     a = 0.1, sigma = 0.01 (test/TestCuda.cu:20-31)."""
@@ -100,3 +123,86 @@ def book(steps=(1, 2, 3, 4, 5)) -> Options:
     return _opts(np.full(n, 63.0), np.full(n, 9.0), np.full(n, 3.0),
                  np.array(steps, np.uint16), np.full(n, 0.1), np.full(n, 0.01),
                  np.full(n, PUT, np.int8))
+
+
+# ------------------------------------------------------------------ reference sets
+REFERENCE_SETS = ("0_UNIFORM", "1_RAND", "2_RANDCONSTHEIGHT", "3_RANDCONSTWIDTH", "4_SKEWED",
+                  "5_SKEWEDCONSTHEIGHT", "6_SKEWEDCONSTWIDTH")
+
+
+@functools.lru_cache(maxsize=None)
+def reference_rate_for_width(width: int, steps: int = 12):
+    """The reversion rate the reference's generator uses for a tree `width`
+    nodes wide at `steps` per year: the smallest rate with five decimals that
+    gives that width (jmax = (int)(0.184 / (1 - e^{-a dt})) + 1,
+    common/OptionConstants.hpp:44-51).  That is the reference's own table
+    (data-generator/RandomDataGenerator.cpp:54-55), entry for entry; None for
+    the widths no five-decimal rate gives (957, 989, 1007 and 1021 at 12 steps),
+    which the table lacks."""
+    J = (int(width) - 1) // 2
+    if width % 2 == 0 or J < 2:
+        return None
+    dt = 1.0 / steps
+    lo = -np.log1p(-0.184 / J) / dt  # rates above lo have jmax <= J
+    a = np.ceil(lo * 1e5 - 1e-6) / 1e5
+    for cand in (a, a + 1e-5):
+        M = np.exp(-cand * dt) - 1.0
+        if 2 * (int(-0.184 / M) + 1) + 1 == width:
+            return float(round(cand, 5))
+    return None
+
+
+def _rate_by_width_range(rng, lo: int, hi: int) -> float:
+    """getRRByWidthRange (RandomDataGenerator.cpp:57-71): a width drawn
+    uniformly from [lo, hi] until one has a rate, then that rate."""
+    while True:
+        w = int(rng.integers(lo, hi + 1))
+        a = reference_rate_for_width(w)
+        if a is not None:
+            return a
+
+
+def reference_set(kind, n: int = 65536, seed: int = 7, skew_percent: int = 1) -> Options:
+    """One of the reference's seven benchmark distributions
+    (RandomDataGenerator.cpp:114-257), shuffled.  Every option is a 3-year put
+    (Lengths 3) on a bond of integer maturity, strike 63, term unit 365,
+    12 steps a year, sigma 0.01; the distributions differ in maturity (height
+    12 T + 1 levels) and width:
+      0_UNIFORM            T = 9, a = 0.1 (width 47)
+      1_RAND               T ~ U{1..100}, width ~ U[7, 511]
+      2_RANDCONSTHEIGHT    T = 9, width ~ U[7, 511]
+      3_RANDCONSTWIDTH     T ~ U{1..100}, a = 0.1
+      4_SKEWED             skew_percent of the options T ~ U{70..100}, width ~ U[411, 511];
+                           the rest T ~ U{1..30}, width ~ U[7, 107]
+      5_SKEWEDCONSTHEIGHT  skew_percent T = 100, width ~ U[7, 107]; the rest as in 4
+      6_SKEWEDCONSTWIDTH   skew_percent T ~ U{1..30} at one width drawn from U[509, 511];
+                           the rest as in 4
+    The skewed part holds ceil(skew_percent/100 * n) options, as the
+    reference's loops count them."""
+    k = REFERENCE_SETS.index(kind) if isinstance(kind, str) else int(kind)
+    rng = np.random.default_rng(seed)
+    nskew = int(np.ceil(skew_percent / 100.0 * n)) if k >= 4 else 0
+    mat = np.empty(n)
+    a = np.empty(n)
+    for i in range(n):
+        if k == 0:
+            mat[i], a[i] = 9, 0.1
+        elif k == 1:
+            mat[i], a[i] = rng.integers(1, 101), _rate_by_width_range(rng, 7, 511)
+        elif k == 2:
+            mat[i], a[i] = 9, _rate_by_width_range(rng, 7, 511)
+        elif k == 3:
+            mat[i], a[i] = rng.integers(1, 101), 0.1
+        elif i >= nskew:
+            mat[i], a[i] = rng.integers(1, 31), _rate_by_width_range(rng, 7, 107)
+        elif k == 4:
+            mat[i], a[i] = rng.integers(70, 101), _rate_by_width_range(rng, 411, 511)
+        elif k == 5:
+            mat[i], a[i] = 100, _rate_by_width_range(rng, 7, 107)
+        else:
+            if i == 0:
+                skew_rate = _rate_by_width_range(rng, 509, 511)
+            mat[i], a[i] = rng.integers(1, 31), skew_rate
+    perm = rng.permutation(n)
+    return _opts(np.full(n, 63.0), mat[perm], np.full(n, 3.0), np.full(n, 12, np.uint16), a[perm],
+                 np.full(n, 0.01), np.full(n, PUT, np.int8))

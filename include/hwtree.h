@@ -111,6 +111,10 @@ typedef struct {
  * wide trees first, then claim the narrow ones in chunks as they run out, so
  * narrow trees fill the wide trees' tail.  Overrides HWT_FLAG_CONCURRENT. */
 #define HWT_FLAG_FUSED 8
+/* Form 2: price trees up to 21 nodes wide in 16-lane rows (or one per
+ * wavefront with HWT_FLAG_NO_ROWS) instead of side by side across a whole
+ * wavefront, one node per lane (the flattened form). */
+#define HWT_FLAG_NO_FLAT 16
 
 typedef struct {
     double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),

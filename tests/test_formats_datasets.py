@@ -79,3 +79,60 @@ def test_box_mix_outside_fraction(pkg):
     assert abs(outside.mean() - 0.05) < 0.002
     inside = ~outside
     assert o.a[inside].max() <= 0.2 and o.length[inside].max() <= 30.0
+
+
+# ------------------------------------------------ the reference's seven datasets
+def _shape(o):
+    tuy = np.ceil(365.0 / o.term_unit)
+    n = (o.term_steps * tuy * o.maturity).astype(int)
+    M = np.exp(-o.a * (tuy / o.term_steps)) - 1.0
+    return 2 * ((-0.184 / M).astype(int) + 1) + 1, n + 1  # width, height in levels
+
+
+# Width and height statistics (mean, variance, min, max) of the reference's
+# 65,536-option data files: thesis/appendices/GeneratedData.tex for sets 0-4;
+# for 5 and 6 the appendix describes an earlier draw, so the figures are those
+# of data/5_SKEWEDCONSTHEIGHT.in and data/6_SKEWEDCONSTWIDTH.in, which follow
+# the generator's code (RandomDataGenerator.cpp:190-255).
+REFERENCE_STATS = {
+    "0_UNIFORM": ((47.0, 0.0, 7, 47), (109.0, 0.0, 109, 109)),
+    "1_RAND": ((259.78, 21293.52, 7, 511), (606.26, 120065.68, 13, 1201)),
+    "2_RANDCONSTHEIGHT": ((258.52, 21358.88, 7, 511), (109.0, 0.0, 109, 109)),
+    "3_RANDCONSTWIDTH": ((47.0, 0.0, 47, 47), (607.04, 119714.09, 13, 1201)),
+    "4_SKEWED": ((60.90, 2490.82, 7, 511), (195.35, 17598.95, 13, 1201)),
+    "5_SKEWEDCONSTHEIGHT": ((57.02, 873.11, 7, 107), (197.43, 20849.10, 13, 1201)),
+    "6_SKEWEDCONSTWIDTH": ((61.56, 2881.24, 7, 511), (186.92, 10793.20, 13, 361)),
+}
+
+
+def test_reference_rate_table_reproduced(pkg):
+    """The generator's width -> reversion-rate rule reproduces the reference's
+    table (RandomDataGenerator.cpp:54-55) entry for entry, and has no rate for
+    exactly the widths the table lacks."""
+    import json
+    t = json.loads((GOLDEN / "width_table.json").read_text())
+    D = pkg.datasets
+    assert all(D.reference_rate_for_width(w) == a for a, w in zip(t["a"], t["width"]))
+    missing = [w for w in range(7, 1024, 2) if D.reference_rate_for_width(w) is None]
+    assert missing == sorted(set(range(7, 1024, 2)) - set(t["width"]))
+
+
+@pytest.mark.parametrize("name", list(REFERENCE_STATS))
+def test_reference_sets_match_the_reference_distributions(pkg, name):
+    o = pkg.datasets.reference_set(name, 65536, seed=3)
+    assert o.N == 65536
+    assert np.all(o.strike == 63.0) and np.all(o.length == 3.0) and np.all(o.term_unit == 365)
+    assert np.all(o.term_steps == 12) and np.all(o.sigma == 0.01) and np.all(o.type == pkg.PUT)
+    w, h = _shape(o)
+    for x, (mean, var, lo, hi) in ((w, REFERENCE_STATS[name][0]), (h, REFERENCE_STATS[name][1])):
+        sd = np.sqrt(var)
+        # one draw against another: 6 standard errors of the mean, 5% of the variance
+        assert abs(x.mean() - mean) <= 6 * sd / np.sqrt(o.N) + 1e-9, (name, x.mean(), mean)
+        assert abs(x.var() - var) <= 0.05 * var + 1e-9, (name, x.var(), var)
+        assert x.max() <= hi and x.min() >= lo, (name, x.min(), x.max())
+    if name.startswith(("4", "5", "6")):
+        # ceil(1% of 65,536) skewed options, as the reference's loop counts them
+        skew = {"4": w >= 411, "5": h == 1201, "6": w >= 509}[name[0]]
+        assert skew.sum() == 656
+        if name.startswith("6"):
+            assert len(set(w[skew])) == 1

@@ -335,8 +335,62 @@ def test_long_curve_on_a_multi_block_batch(pkg, oracle):
     assert_parity(got[sub], oracle.price(opts.take(sub), dense), "3,650-knot curve")
 
 
+@pytest.mark.parametrize("name", ["0_UNIFORM", "1_RAND", "2_RANDCONSTHEIGHT", "3_RANDCONSTWIDTH", "4_SKEWED",
+                                  "5_SKEWEDCONSTHEIGHT", "6_SKEWEDCONSTWIDTH"])
+def test_reference_distributions_65k_stratified(pkg, oracle, golden_yield, name):
+    """The reference's seven 65,536-option benchmark distributions, drawn by
+    the seeded generator and priced whole on the reference's 15-knot curve:
+    a stratified subset (every tree width and height class, the skewed part
+    included) against the oracle, and the whole batch against the
+    one-tree-per-wave arrangement."""
+    opts = pkg.datasets.reference_set(name, 65536, seed=5)
+    got = pkg.capi.price(opts, golden_yield)
+    assert np.all(np.isfinite(got))
+    w = 2 * ((-0.184 / (np.exp(-opts.a / 12.0) - 1.0)).astype(int) + 1) + 1
+    order = np.lexsort((opts.maturity, w))
+    sub = np.unique(np.concatenate([order[::331], order[-40:], np.nonzero(w >= 411)[0][:40]]))
+    assert_parity(got[sub], oracle.price(opts.take(sub), golden_yield), name)
+    other = pkg.capi.price(opts, golden_yield, pkg.capi.config(flags=pkg.capi.FLAG_NO_ROWS | pkg.capi.FLAG_FUSED))
+    assert_parity(other, got, name + " (one tree per wave, fused)")
+
+
+@pytest.mark.parametrize("flags", [0, 2, 4, 8, 16])
+def test_flat_form_tiny_trees(pkg, oracle, golden_yield, flags):
+    """Trees 3-21 nodes wide, packed 64 / w to a wavefront one node per lane
+    (the flattened form; flags 16 turns it off, 2 puts the narrow trees one
+    per wavefront, 4 and 8 run the narrow kernel beside or fused with the
+    wide one), with the extreme shapes: jmax = 1, exercise at step 0, a
+    one-step tree, a mix of heights in every group."""
+    y = golden_yield
+    opts = pkg.datasets.tiny(61, 5000)
+    extra = pkg.Options(strike=[63.0, 70.0, 80.0, 90.0], maturity=[1.0 / 12, 0.25, 9.0, 30.0],
+                        length=[0.0, 0.0, 3.0, 29.9], term_unit=[365] * 4, term_steps=[12] * 4,
+                        a=[3.0, 0.6, 3.0, 0.3], sigma=[0.01, 0.02, 0.01, 0.005], type=[1, 0, 1, 0])
+    opts = pkg.Options.concat([extra, opts])
+    got = pkg.capi.price(opts, y, pkg.capi.config(flags=flags))
+    sub = np.concatenate([np.arange(4), np.arange(4, opts.N, 7)])
+    assert_parity(got[sub], oracle.price(opts.take(sub), y), f"flags {flags}")
+    assert_parity(got, pkg.capi.price(opts, y), f"flags {flags} against the default")
+
+
+def test_flat_form_mixed_precision_and_packing(pkg, oracle, golden_yield):
+    """Mixed precision through the flattened form, within the mixed bar of the
+    oracle; prices do not depend on which trees share a wavefront (the same
+    options shuffled price the same, bit for bit)."""
+    y = golden_yield
+    opts = pkg.datasets.tiny(67, 3000)
+    mixed = pkg.capi.price(opts, y, pkg.capi.config(precision=pkg.capi.PREC_MIXED))
+    sub = np.arange(0, opts.N, 11)
+    assert np.abs(mixed[sub] - oracle.price(opts.take(sub), y)).max() <= 1e-3
+    a = pkg.capi.price(opts, y)
+    perm = np.random.default_rng(2).permutation(opts.N)
+    b = pkg.capi.price(opts.take(perm), y)
+    assert np.array_equal(a[perm], b)
+
+
 @pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"flags": 8}, {"flags": 10},
-                                     {"flags": 9}, {"wide_per_cu": 2, "narrow_per_cu": 4},
+                                     {"flags": 9}, {"flags": 16}, {"flags": 18}, {"flags": 24},
+                                     {"wide_per_cu": 2, "narrow_per_cu": 4},
                                      {"sort": "H"}, {"sort": "W"}])
 def test_form2_settings_price_the_same(pkg, oracle, setting):
     """Every setting the autotuner can choose prices within the parity bar."""
