@@ -121,20 +121,52 @@ HWT_HD void edge_table_general(const TreeShape &t, double *tab)
     }
 }
 
+// Node discounts of the four bottom and four top nodes (E0..E3 of nodes 0..3,
+// Ea..Ed of nodes w-1..w-4) from four exponentials instead of eight: with
+// h = dr dt and X = jmax h, E0 = e^X and E_{k+1} = E_k e^{-h} going up,
+// Ea = e^{-X} and E_{k-1} = E_k e^{h} going down (the pricers advance their
+// interior discounts the same way, one multiplication per node).
+struct EdgeDiscounts {
+    double E0, E1, E2, E3, Ea, Eb, Ec, Ed;
+};
+
+HWT_HD EdgeDiscounts edge_discounts(const TreeShape &t)
+{
+    const double h = t.dr * t.dt, X = t.jmax * h;
+    const double q = exp(-h), qi = exp(h);
+    EdgeDiscounts e;
+    e.E0 = exp(X);
+    e.E1 = e.E0 * q;
+    e.E2 = e.E1 * q;
+    e.E3 = e.E2 * q;
+    e.Ea = exp(-X);
+    e.Eb = e.Ea * qi;
+    e.Ec = e.Eb * qi;
+    e.Ed = e.Ec * qi;
+    return e;
+}
+
+// Edge-elimination factors from those discounts (as edge_factors, trees of
+// at least 5 nodes): rho = F[2][0] / F[2][1], rhop = F[w-3][w-1] / F[w-3][w-2].
+HWT_HD void edge_factors(TreeShape &t, const EdgeDiscounts &e)
+{
+    const int J = t.jmax;
+    t.rho = (pu_bot(-J, t.M) * e.E0) / (pu_std(1 - J, t.M) * e.E1);
+    t.rhop = (pd_top(J, t.M) * e.Ea) / (pd_std(J - 1, t.M) * e.Eb);
+}
+
 // Closed form for trees at least 7 nodes wide, where the two edge
 // corrections touch disjoint rows and columns.  With F[r][s] = p(s->r) E_s,
 // bottom rho = F20/F21, top rhop = F[c][a]/F[c][b] (a = w-1, b = w-2, c = w-3):
 //   G00 = F00 - rho F01,  G10 = F10 - rho F11 + rho G00,  G11 = F11 + rho F01,
 //   G[a][a] = Faa - rhop Fab,  G[b][a] = Fba - rhop Fbb + rhop G[a][a],
 //   G[b][b] = Fbb + rhop Fab,  every other entry equals F.
-HWT_HD void edge_table_closed(const TreeShape &t, double *tab)
+HWT_HD void edge_table_closed(const TreeShape &t, const EdgeDiscounts &e, double *tab)
 {
     const double M = t.M, rho = t.rho, rhop = t.rhop;
-    const int J = t.jmax, a = t.w - 1, b = t.w - 2, c = t.w - 3, d = t.w - 4;
-    const double E0 = node_disc(0, J, t.dr, t.dt), E1 = node_disc(1, J, t.dr, t.dt);
-    const double E2 = node_disc(2, J, t.dr, t.dt), E3 = node_disc(3, J, t.dr, t.dt);
-    const double Ea = node_disc(a, J, t.dr, t.dt), Eb = node_disc(b, J, t.dr, t.dt);
-    const double Ec = node_disc(c, J, t.dr, t.dt), Ed = node_disc(d, J, t.dr, t.dt);
+    const int J = t.jmax, b = t.w - 2, c = t.w - 3, d = t.w - 4;
+    const double E0 = e.E0, E1 = e.E1, E2 = e.E2, E3 = e.E3;
+    const double Ea = e.Ea, Eb = e.Eb, Ec = e.Ec, Ed = e.Ed;
     // bottom: node 0 branches (j=-J) to 0,1,2; nodes 1,2,3 are standard
     const double F00 = pd_bot(-J, M) * E0, F10 = pm_bot(-J, M) * E0;
     const double F01 = pd_std(1 - J, M) * E1, F11 = pm_std(1 - J, M) * E1, F21 = pu_std(1 - J, M) * E1;
@@ -164,7 +196,7 @@ HWT_HD void edge_table_closed(const TreeShape &t, double *tab)
 HWT_HD void edge_table(const TreeShape &t, double *tab)
 {
     if (t.w < 7) edge_table_general(t, tab);
-    else edge_table_closed(t, tab);
+    else edge_table_closed(t, edge_discounts(t), tab);
 }
 
 __device__ __forceinline__ void load_edge_table(const double *src, int lane, lds_f64 *tab)
@@ -343,80 +375,102 @@ __device__ __forceinline__ void init_window(T (&z)[K], const TreeShape &t, int b
 // zero coefficients.  The setup kernel checks per option that every scale
 // stays in range (unit_form_ok); the plain stencil prices the rest.
 //
-// The plain coefficients come from window_coefs (edge nodes patched by a
-// wave-uniform switch, so register indices stay static) and are transformed
-// in place; z is scaled with one division per lane.  Undoing D after the
-// steps needs the scales again: in FP64 each lane parks them in its own node
-// slots of the LDS state buffer, which the steps leave untouched; in mixed
-// mode (an FP32 buffer) they are recomputed.
+// The coefficients are formed one node at a time and transformed on the
+// way (the closed forms, with edge nodes read from the LDS table at a
+// per-lane index), so no plain coefficient array is ever live beside the
+// transformed ones: the set-up needs no more registers than the steps, and
+// the form fits every window size at two waves per SIMD.  Undoing D after the
+// steps needs the scales again: each lane parks them in its own node slots of
+// the LDS state buffer, which the steps leave untouched.  FP64 only.
+
+// Three-point coefficients of node k (closed forms; edge nodes from the
+// table; zero past the top edge).  Eprev, Ek, Enext are the discounts of nodes
+// k-1, k, k+1, already zero outside the tree.
+template <bool FWD>
+__device__ __forceinline__ void node_coefs(const TreeShape &t, int k, double Eprev, double Ek, double Enext,
+                                           const lds_f64 *tab, double &cl, double &cm, double &cr)
+{
+    const int j = k - t.jmax;
+    if (FWD) {
+        cl = pu_std(j - 1, t.M) * Eprev;
+        cm = pm_std(j, t.M) * Ek;
+        cr = pd_std(j + 1, t.M) * Enext;
+    } else {
+        cl = pd_std(j, t.M) * Ek;
+        cm = pm_std(j, t.M) * Ek;
+        cr = pu_std(j, t.M) * Ek;
+    }
+    if (k >= t.w) cl = cm = cr = 0.0;
+    if (k < 3 || (k >= t.w - 3 && k < t.w)) {
+        const int e = 3 * edge_slot_of(k, t.w);
+        cl = tab[e];
+        cm = tab[e + 1];
+        cr = tab[e + 2];
+    }
+}
 
 // Transformed coefficients of this lane's nodes, and z -> D^{-1} z: the chain
-// d_s = c_s d_{s-1} is built going up, then walked down with
-// 1/d_{s-1} = c_s / d_s.
-template <class T, int K, bool FWD>
-__device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, const lds_f64 *etab, T (&z)[K],
-                                        T &cl0, T (&um)[K], T (&ur)[K], typename Lds<T>::type *lds)
+// d_s = c_s d_{s-1} (c_s = cl_s on tree nodes, 1 on padding) is built going
+// up with cr'_{s-1} = cr_{s-1} c_s, then walked down with 1/d_{s-1} = c_s / d_s.
+template <int K, bool FWD>
+__device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, const lds_f64 *etab,
+                                        double (&z)[K], double &cl0, double (&um)[K], double (&ur)[K], lds_f64 *lds)
 {
-    double cl[K], cm[K], cr[K];
-    window_coefs<double, K, FWD>(t, base, lane, etab, cl, cm, cr);
+    const lds_f64 *tab = etab + (FWD ? 0 : 18);
     const int k0 = base + lane * K;
-    double d = 1.0;
-#pragma unroll
-    for (int s = 1; s < K; ++s) {
-        cl[s] = (k0 + s < t.w) ? cl[s] : 1.0;
-        d *= cl[s];
-        cr[s - 1] *= cl[s];
-        if constexpr (sizeof(T) == sizeof(double)) lds[k0 + s] = d;
-    }
-    double inv = 1.0 / d;
-    cr[K - 1] *= inv;
-    cl0 = (T)(cl[0] * dpp_from_lower_lane(d));
-#pragma unroll
-    for (int s = K - 1; s > 0; --s) {
-        z[s] = (T)((double)z[s] * inv);
-        inv *= cl[s];
-    }
+    const double q = exp(-t.dr * t.dt);
+    auto inside = [&](int k) { return k >= 0 && k < t.w; };
+    double E = node_disc(k0 - 1, t.jmax, t.dr, t.dt);  // discount of node k0-1+m at step m
+    double Eprev = inside(k0 - 1) ? E : 0.0;
+    E *= q;
+    double Ek = inside(k0) ? E : 0.0;
+    double c[K];  // the chain factors, kept for the walk down
+    double cl, cm, cr, cr_prev = 0.0, d = 1.0;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-        um[s] = (T)cm[s];
-        ur[s] = (T)cr[s];
+        const int k = k0 + s;
+        E *= q;
+        const double Enext = inside(k + 1) ? E : 0.0;
+        node_coefs<FWD>(t, k, Eprev, Ek, Enext, tab, cl, cm, cr);
+        um[s] = cm;
+        if (s == 0) {
+            cl0 = cl;
+        } else {
+            c[s] = (k < t.w) ? cl : 1.0;
+            d *= c[s];
+            ur[s - 1] = cr_prev * c[s];
+            lds[k] = d;
+        }
+        cr_prev = cr;
+        Eprev = Ek;
+        Ek = Enext;
+    }
+    double inv = 1.0 / d;
+    ur[K - 1] = cr_prev * inv;
+    cl0 *= dpp_from_lower_lane(d);
+#pragma unroll
+    for (int s = K - 1; s > 0; --s) {
+        z[s] *= inv;
+        inv *= c[s];
     }
 }
 
-// z -> D z with the scales unit_in used.  When they are recomputed, the
-// window base passes through an empty asm so the compiler cannot merge this
-// with unit_in's computation and keep that alive across the steps.
-template <class T, int K, bool FWD>
-__device__ __forceinline__ void unit_out(const TreeShape &t, int base, int lane, const lds_f64 *etab, T (&z)[K],
-                                         typename Lds<T>::type *lds)
+// z -> D z with the scales unit_in parked in LDS.
+template <int K>
+__device__ __forceinline__ void unit_out(int base, int lane, double (&z)[K], const lds_f64 *lds)
 {
     const int k0 = base + lane * K;
-    if constexpr (sizeof(T) == sizeof(double)) {
 #pragma unroll
-        for (int s = 1; s < K; ++s) z[s] = (T)((double)z[s] * lds[k0 + s]);
-    } else {
-        asm volatile("" : "+s"(base));
-        double cl[K], cm[K], cr[K];
-        window_coefs<double, K, FWD>(t, base, lane, etab, cl, cm, cr);
-        double d = 1.0;
-#pragma unroll
-        for (int s = 1; s < K; ++s) {
-            d *= (base + lane * K + s < t.w) ? cl[s] : 1.0;
-            z[s] = (T)((double)z[s] * d);
-        }
-    }
+    for (int s = 1; s < K; ++s) z[s] *= lds[k0 + s];
 }
 
-// Largest window the unit form is used for.  FP64: 12 nodes per lane.
-// Windows of K <= 12 carry about 80% of the stencil work of a typical wide
-// batch; limits of 14 and 16 still fit two waves per SIMD with a few more
-// spilled registers but run no faster, and from 18 on the spills grow into
-// the hundreds.  FP32 (mixed mode): not used -- the undiscounted
+// Largest window the unit form is used for.  FP64: every window (21 nodes
+// per lane).  FP32 (mixed mode): not used -- the undiscounted
 // backward state of a long, weakly mean-reverting bond already comes within a
 // few powers of ten of the FP32 limit (about 1e34 for a 100-year bond with
 // a = 0.009), and D^{-1} would scale it past.
 template <class T> struct UnitMaxK;
-template <> struct UnitMaxK<double> { static constexpr int value = 12; };
+template <> struct UnitMaxK<double> { static constexpr int value = MAXK; };
 template <> struct UnitMaxK<float> { static constexpr int value = 1; };
 
 // a*b + c into a fresh register.  Written as the three-address v_fma_f64:
@@ -478,10 +532,23 @@ template <> struct UnitRange<float> { static constexpr double dmin = 1e-30; };
 // discount in [e^{-X}, e^{X}], X = jmax dr dt, and the edge entries are the
 // table's sub-diagonals of nodes 1, 2, w-3, w-2, w-1 (node 0 always starts a
 // lane).
-HWT_HD bool unit_form_ok(const TreeShape &t, const double *tab, double dmin)
+
+// x^f for a small count f >= 0, by repeated squaring.
+HWT_HD double pow_count(double x, int f)
 {
-    const double jm = t.jmax * fabs(t.M), X = t.jmax * t.dr * t.dt;
-    double lo = (1.0 / 6.0 - 0.5 * jm) * exp(-X), hi = (1.0 / 6.0 + 0.5 * (jm * jm + jm)) * exp(X);
+    double r = 1.0;
+    for (; f > 0; f >>= 1, x *= x)
+        if (f & 1) r *= x;
+    return r;
+}
+
+// eX = e^{X} and emX = e^{-X} are the discounts of nodes 0 and w-1.  The range
+// test lo^f >= dmin, hi^f <= 1/dmin is taken on powers rather than logarithms:
+// an underflow or overflow on the way only makes it fail.
+HWT_HD bool unit_form_ok(const TreeShape &t, const double *tab, double dmin, double eX, double emX)
+{
+    const double jm = t.jmax * fabs(t.M);
+    double lo = (1.0 / 6.0 - 0.5 * jm) * emX, hi = (1.0 / 6.0 + 0.5 * (jm * jm + jm)) * eX;
     for (int pass = 0; pass < 2; ++pass)
         for (int slot = 1; slot < 6; ++slot) {
             const double a = fabs(tab[18 * pass + 3 * slot]);
@@ -490,8 +557,13 @@ HWT_HD bool unit_form_ok(const TreeShape &t, const double *tab, double dmin)
         }
     if (!(lo > 0.0) || !(hi < 1e300)) return false;
     const int f = (t.w + WAVE - 1) / WAVE - 1;  // most factors in one lane
-    const double L = f * log(lo < 1.0 ? lo : 1.0), U = f * log(hi > 1.0 ? hi : 1.0);
-    return L >= log(dmin) && U <= -log(dmin);
+    return pow_count(lo < 1.0 ? lo : 1.0, f) >= dmin && pow_count(hi > 1.0 ? hi : 1.0, f) <= 1.0 / dmin;
+}
+
+HWT_HD bool unit_form_ok(const TreeShape &t, const double *tab, double dmin)
+{
+    const double X = t.jmax * t.dr * t.dt;
+    return unit_form_ok(t, tab, dmin, exp(X), exp(-X));
 }
 
 // One phase: bring the window in (initial values, or the previous window's
@@ -513,12 +585,17 @@ __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nstep
     }
     __syncthreads();
     if (nsteps > 0) {
-        if (K > 1 && K <= UnitMaxK<T>::value && unit) {
-            T cl0, um[K], ur[K];
-            unit_in<T, K, FWD>(t, base, lane, etab, z, cl0, um, ur, lds);
-            unit_steps<T, K>(z, cl0, um, ur, nsteps);
-            unit_out<T, K, FWD>(t, base, lane, etab, z, lds);
-        } else {
+        bool done = false;
+        if constexpr (K > 1 && K <= UnitMaxK<T>::value) {
+            if (unit) {
+                double cl0, um[K], ur[K];
+                unit_in<K, FWD>(t, base, lane, etab, z, cl0, um, ur, lds);
+                unit_steps<double, K>(z, cl0, um, ur, nsteps);
+                unit_out<K>(base, lane, z, lds);
+                done = true;
+            }
+        }
+        if (!done) {
             T cl[K], cm[K], cr[K];
             window_coefs<T, K, FWD>(t, base, lane, etab, cl, cm, cr);
             stencil_steps<T, K>(z, cl, cm, cr, nsteps);

@@ -299,23 +299,26 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 const double te = c.e * c.dt;
                 c.Pe = exp(-yield(te, tu) * te);
                 TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, 0.0, 0.0};
-                edge_factors(t);
-                c.rho = t.rho;
-                c.rhop = t.rhop;
-                // trees of 7 or more nodes (all but the tiniest) build the
-                // table in registers, store it, and check it from the registers
+                // trees of 7 or more nodes (all but the tiniest) take their
+                // edge discounts from four exponentials, build the table in
+                // registers, store it, and check it from the registers
                 double *eo = edge + (size_t)i * EDGE_TAB;
                 if (t.w >= 7) {
+                    const EdgeDiscounts ed = edge_discounts(t);
+                    edge_factors(t, ed);
                     double tab[EDGE_TAB];
-                    edge_table_closed(t, tab);
+                    edge_table_closed(t, ed, tab);
 #pragma unroll
                     for (int k = 0; k < EDGE_TAB / 2; ++k)
                         reinterpret_cast<double2 *>(eo)[k] = make_double2(tab[2 * k], tab[2 * k + 1]);
-                    c.unit = unit_form_ok(t, tab, unit_dmin) ? 1 : 0;
+                    c.unit = unit_form_ok(t, tab, unit_dmin, ed.E0, ed.Ea) ? 1 : 0;
                 } else {
+                    edge_factors(t);
                     edge_table_general(t, eo);
                     c.unit = unit_form_ok(t, eo, unit_dmin) ? 1 : 0;
                 }
+                c.rho = t.rho;
+                c.rhop = t.rhop;
                 c.work = dual_node_updates(c.n, c.e, c.jmax);
                 const int64_t seqw = seq_node_updates(c.n, c.jmax);
                 atomicAdd(&lsum[0], (unsigned long long)c.work);
@@ -366,12 +369,13 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_
     const int i = tid + blockIdx.x * SETUP_BLOCK;
     if (count) n = min(*count, n);
     const int b = (i < n) ? bucket[i] : -1;
+    const int ob = (b >= 0) ? offs[b] : 0;  // requested before the barriers, so it arrives meanwhile
     const int r = (b >= 0) ? atomicAdd(&lcount[b], 1) : 0;
     __syncthreads();
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
         if (lcount[k]) lbase[k] = atomicAdd(&cursor[k], lcount[k]);
     __syncthreads();
-    if (b >= 0) order[offs[b] + lbase[b] + r] = i;
+    if (b >= 0) order[ob + lbase[b] + r] = i;
 }
 
 // Form 1: one lane per option over every scheduled task (tasks ordered by
