@@ -320,6 +320,11 @@ def main():
     if tuned.form == capi.FORM_LANE:  # the inspector chose form 1: one pricing kernel
         wide_ms, wide_flop, dominant = float(np.mean(kern)), FLOP_PER_NODE_UPDATE * s["node_updates"], "k_price_lane"
     achieved = wide_flop / (wide_ms * 1e-3) / 1e12
+    # what the kernel's stencil steps executed, counted by the kernel itself
+    # (padding slots included; 4K+1 operations per lane and step in the
+    # unit-diagonal form, 5K in the plain one)
+    exec_ops = s["wide_step_ops"] if dominant == "k_price_wide" else None
+    slots = s["wide_unit_slots"] + s["wide_plain_slots"]
 
     # the drop-in one-shot call on host arrays (plan creation, H2D, pricing,
     # D2H, plan destruction): the PCIe-inclusive rate, reported beside `value`
@@ -368,6 +373,16 @@ def main():
             "flop_per_launch": wide_flop,
             "flop_model": "5 FP64 FLOP per stencil node update (mul + 2 fma), node updates counted over "
                           "the active tree region",
+            "executed": None if exec_ops is None else {
+                "flop_per_launch": exec_ops,
+                "achieved": exec_ops / (wide_ms * 1e-3) / 1e12,
+                "frac": exec_ops / (wide_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "unit_form_slots": s["wide_unit_slots"],
+                "plain_form_slots": s["wide_plain_slots"],
+                "slots_per_counted_update": slots / s["wide_node_updates"],
+                "note": "FP64 operations the kernel's stencil steps executed, counted by the kernel (window "
+                        "padding included): 4K+1 per lane and step in the unit-diagonal form, 5K in the plain form",
+            },
         },
         "detail": {
             "node_updates": st["node_updates"],

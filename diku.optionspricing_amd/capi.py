@@ -89,7 +89,8 @@ class StatsC(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("wide_ms", C.c_double),
                 ("narrow_ms", C.c_double), ("device_memory", C.c_int64), ("node_updates", C.c_int64),
                 ("wide_node_updates", C.c_int64), ("seq_node_updates", C.c_int64), ("tasks", C.c_int32),
-                ("wide_tasks", C.c_int32), ("unsupported", C.c_int32), ("fp64_repriced", C.c_int32)]
+                ("wide_tasks", C.c_int32), ("unsupported", C.c_int32), ("fp64_repriced", C.c_int32),
+                ("wide_step_ops", C.c_int64), ("wide_unit_slots", C.c_int64), ("wide_plain_slots", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

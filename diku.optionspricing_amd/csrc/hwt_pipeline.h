@@ -87,7 +87,12 @@ enum Tally {
     // first, as the scan lays them out), then the end of the last part
     T_FLAT_START = 32,  // FLAT_PARTS + 1 entries
     T_FLAT_GROUP = 48,  // FLAT_PARTS + 1 entries
-    T_ALLOC = 64
+    // int64: what the windowed kernels' stencil steps executed (StepCount):
+    // operations, unit-form node slots, plain-form node slots
+    T_XFLOP = 64,
+    T_XUNIT = 66,
+    T_XPLAIN = 68,
+    T_ALLOC = 72
 };
 
 // Pricing kernels (hwt_pricers.h).  All are persistent: one single-wave

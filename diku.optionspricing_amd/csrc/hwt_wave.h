@@ -654,11 +654,14 @@ __device__ __forceinline__ Window window_for(int D, const TreeShape &t)
     return Window{(t.w + WAVE - 1) / WAVE, 0};
 }
 
-// Forward pass: state 0 (delta at the root) -> state e.  The state ends up in
-// out[lo, hi) (global scratch); tree nodes outside that range are zero.
-template <class T, int KMAX, int G>
-__device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, typename Lds<T>::type *lds,
-                                             const lds_f64 *etab, T *out, int &lo, int &hi, bool unit)
+// Window schedules of the two passes: the phases (window, steps) in order.
+// The pricing passes below run a phase at each; the step counter at sync
+// time (k_count_steps) walks the same schedules to count what they executed.
+// visit(win, nsteps, in_lo, in_hi, last): in_lo/in_hi bound the previous
+// window's tree nodes (empty for the first phase); last marks the forward
+// pass's final phase.
+template <int G, class Visit>
+__device__ __forceinline__ void forward_schedule(const TreeShape &t, int e, int &lo, int &hi, Visit &&visit)
 {
     Window win = window_for<G>(0, t);
     int in_lo = 0, in_hi = 0;  // empty: initial values
@@ -667,8 +670,7 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
         const bool full = 32 * win.K > t.jmax;  // growth windows satisfy 32K <= jmax
         const int iend = full ? e : min(e, 32 * win.K - 1);
         const bool last = iend >= e;
-        dispatch_phase<T, KMAX, true>(win.K, t, win.base, iend - i, in_lo, in_hi, lane, lds, etab,
-                                      last ? out : nullptr, unit);
+        visit(win, iend - i, in_lo, in_hi, last);
         i = iend;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -679,12 +681,10 @@ __device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane
     hi = in_hi;
 }
 
-// Backward pass: state n (bond face value 100) -> state e, keeping only the
-// nodes that can still reach the exercise nodes |j| <= min(e, jmax).  The
-// state ends up in lds[lo, hi).
-template <class T, int KMAX, int G>
-__device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, typename Lds<T>::type *lds,
-                                              const lds_f64 *etab, int &lo, int &hi, bool unit)
+// The backward pass keeps only the nodes that can still reach the exercise
+// nodes |j| <= min(e, jmax).
+template <int G, class Visit>
+__device__ __forceinline__ void backward_schedule(const TreeShape &t, int n, int e, int &lo, int &hi, Visit &&visit)
 {
     const int ep = min(e, t.jmax);
     const int Kgmax = t.jmax / 32 / G * G;  // the largest window below the full one (0: none)
@@ -695,8 +695,7 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
         const bool full = 32 * win.K > t.jmax;
         const int klo = full ? Kgmax : win.K - G;  // the next window down
         const int send = max(e, e + 32 * klo - 1 - ep);
-        dispatch_phase<T, KMAX, false>(win.K, t, win.base, s - send, in_lo, in_hi, lane, lds, etab, (T *)nullptr,
-                                       unit);
+        visit(win, s - send, in_lo, in_hi, false);
         s = send;
         in_lo = win.base;
         in_hi = min(win.base + WAVE * win.K, t.w);
@@ -704,6 +703,37 @@ __device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, 
     }
     lo = in_lo;
     hi = in_hi;
+}
+
+// Whether a phase of window size K runs the unit-diagonal form.
+template <class T>
+__device__ __forceinline__ bool unit_phase(int K, bool unit)
+{
+    return unit && K > 1 && K <= UnitMaxK<T>::value;
+}
+
+// Forward pass: state 0 (delta at the root) -> state e.  The state ends up in
+// out[lo, hi) (global scratch); tree nodes outside that range are zero.
+template <class T, int KMAX, int G>
+__device__ __forceinline__ void forward_pass(const TreeShape &t, int e, int lane, typename Lds<T>::type *lds,
+                                             const lds_f64 *etab, T *out, int &lo, int &hi, bool unit)
+{
+    forward_schedule<G>(t, e, lo, hi, [&](const Window &win, int nsteps, int in_lo, int in_hi, bool last) {
+        dispatch_phase<T, KMAX, true>(win.K, t, win.base, nsteps, in_lo, in_hi, lane, lds, etab,
+                                      last ? out : nullptr, unit);
+    });
+}
+
+// Backward pass: state n (bond face value 100) -> state e.  The state ends up
+// in lds[lo, hi).
+template <class T, int KMAX, int G>
+__device__ __forceinline__ void backward_pass(const TreeShape &t, int n, int e, int lane, typename Lds<T>::type *lds,
+                                              const lds_f64 *etab, int &lo, int &hi, bool unit)
+{
+    backward_schedule<G>(t, n, e, lo, hi, [&](const Window &win, int nsteps, int in_lo, int in_hi, bool) {
+        dispatch_phase<T, KMAX, false>(win.K, t, win.base, nsteps, in_lo, in_hi, lane, lds, etab, (T *)nullptr,
+                                       unit);
+    });
 }
 
 // FP64 accumulators for the duality sums: plain in FP64 mode; compensated

@@ -209,6 +209,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
         tally[T_UNSUP] = tally[A_UNSUP];
         tally[T_NREDO] = 0;
         tally[T_QUEUE_X] = 0;
+        for (int k = T_XFLOP; k < T_XPLAIN + 2; ++k) tally[k] = 0;
         t64[A_WORK / 2] = 0;
         t64[A_SEQWORK / 2] = 0;
         t64[A_WORKW / 2] = 0;
@@ -378,6 +379,42 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_
     if (b >= 0) order[ob + lbase[b] + r] = i;
 }
 
+// What the wide-tree kernel's stencil steps executed in the last execution,
+// padding slots included, from the window schedule it walks
+// (forward_schedule / backward_schedule): operations (per lane and step 4K+1
+// in the unit-diagonal form, 5K in the plain one) and node slots (64 K per
+// step) per form.  One thread per wide task; run at sync time, never in the
+// timed pipeline.
+__global__ __launch_bounds__(256) void k_count_steps(const OptConst *__restrict__ cst,
+                                                     const int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                                                     int32_t mixed)
+{
+    __shared__ unsigned long long part[3];
+    if (threadIdx.x < 3) part[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tally[T_NLARGE]) {
+        const OptConst c = cst[order[i]];
+        const TreeShape t{c.w, c.jmax, c.M, c.dr, c.dt, c.rho, c.rhop};
+        unsigned long long ops = 0, slots[2] = {0, 0};
+        auto count = [&](const Window &win, int nsteps, int, int, bool) {
+            const bool u = mixed ? unit_phase<float>(win.K, c.unit != 0) : unit_phase<double>(win.K, c.unit != 0);
+            const unsigned long long nn = (unsigned long long)nsteps * WAVE;
+            ops += nn * (u ? 4 * win.K + 1 : 5 * win.K);
+            slots[u ? 0 : 1] += nn * win.K;
+        };
+        int lo, hi;
+        forward_schedule<WIDE_WINDOW_STEP>(t, c.e, lo, hi, count);
+        backward_schedule<WIDE_WINDOW_STEP>(t, c.n, c.e, lo, hi, count);
+        atomicAdd(&part[0], ops);
+        atomicAdd(&part[1], slots[0]);
+        atomicAdd(&part[2], slots[1]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 && part[threadIdx.x])
+        atomicAdd(reinterpret_cast<unsigned long long *>(tally + T_XFLOP) + threadIdx.x, part[threadIdx.x]);
+}
+
 // Form 1: one lane per option over every scheduled task (tasks ordered by
 // decreasing work, so the lanes of a wave carry similar trees).
 __global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__restrict__ edge,
@@ -525,6 +562,8 @@ struct hwt_plan {
     hipStream_t last_stream = nullptr;
     bool timed_kernel = false;  // evk0/evk1 were recorded by the last execution
     bool timed_split = false;   // so were evw0/evw1 and evn0/evn1
+    int32_t last_n = 0;         // options bound of the last execution
+    bool steps_counted = true;  // k_count_steps has run for the last execution
 };
 
 static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, double *out, hipStream_t s, bool time_kernel)
@@ -769,9 +808,9 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
         hipMemsetAsync(p->tally, 0, T_ALLOC * sizeof(int32_t), p->stream) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess)
         return bail(HWT_EHIP, "clearing the plan's counters failed");
-    if (hipHostMalloc((void **)&p->h_tally, T_NCOUNT_ALL * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&p->h_tally, T_ALLOC * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
-    std::memset(p->h_tally, 0, T_NCOUNT_ALL * sizeof(int32_t));  // a plan never executed syncs clean
+    std::memset(p->h_tally, 0, T_ALLOC * sizeof(int32_t));  // a plan never executed syncs clean
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
         hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
         hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||
@@ -811,6 +850,8 @@ void hwt_plan_destroy(hwt_plan *p)
 static int plan_execute_views(hwt_plan *p, const DevOpts &o, const DevYield &y, double *dout, hipStream_t s)
 {
     p->last_stream = s;
+    p->last_n = o.n;
+    p->steps_counted = false;
     HWT_HIP(hipEventRecord(p->ev0, s));
     if (p->cfg.flags & HWT_FLAG_NO_GRAPH) {
         const int rc = launch_pipeline(p, o, y, dout, s, true);
@@ -878,6 +919,8 @@ int hwt_plan_execute_counted(hwt_plan *p, const hwt_options_f64 *dopts, const in
     o.count = dev_count;
     const DevYield y{dyield->n, dyield->rate, dyield->day};
     p->last_stream = s;
+    p->last_n = o.n;
+    p->steps_counted = false;
     HWT_HIP(hipEventRecord(p->ev0, s));
     const int rc = launch_pipeline(p, o, y, dout, s, true);
     if (rc) return rc;
@@ -889,11 +932,20 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
 {
     if (!p) return fail(HWT_EINVAL, "hwt_plan_sync: null plan");
     hipStream_t s = p->last_stream ? p->last_stream : p->stream;
+    // with stats: what the windowed kernel's steps executed in the last
+    // execution, counted once from its schedule (form 2 only)
+    if (stats && !p->steps_counted && p->last_n > 0 && p->cfg.form != HWT_FORM_LANE) {
+        (void)hipGetLastError();
+        k_count_steps<<<(p->last_n + 255) / 256, 256, 0, s>>>(p->cst, p->order, p->tally,
+                                                             p->cfg.precision == HWT_PREC_MIXED ? 1 : 0);
+        HWT_HIP(hipGetLastError());
+        p->steps_counted = true;
+    }
     HWT_HIP(hipStreamSynchronize(s));
     // the tallies are fetched here, once, rather than by a copy queued behind
     // every execution: back-to-back executions then run without a transfer
     // between them
-    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_NCOUNT_ALL * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_ALLOC * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HWT_HIP(hipStreamSynchronize(s));
     const int32_t *t = p->h_tally;
     if (stats) {
@@ -920,6 +972,9 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
         stats->wide_tasks = t[T_NLARGE];
         stats->unsupported = t[T_UNSUP];
         stats->fp64_repriced = t[T_NREDO];
+        std::memcpy(&stats->wide_step_ops, t + T_XFLOP, sizeof(int64_t));
+        std::memcpy(&stats->wide_unit_slots, t + T_XUNIT, sizeof(int64_t));
+        std::memcpy(&stats->wide_plain_slots, t + T_XPLAIN, sizeof(int64_t));
     }
     if (t[T_INVALID] > 0) return fail(HWT_EINVAL, std::to_string(t[T_INVALID]) + " option(s) with invalid parameters (price NaN)");
     if (t[T_UNSUP] > 0)

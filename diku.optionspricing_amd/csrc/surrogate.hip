@@ -559,7 +559,15 @@ struct InferArgs {
     const int32_t *row_count; // number of listed rows (with `rows`)
 };
 
-__device__ inline uint32_t pack_bf16x2(float a, float b) { return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16); }
+// Two floats to a packed BF16 pair (a in the low half), round to nearest
+// even, as one v_cvt_pk_bf16_f32 (the value-by-value form costs a convert,
+// a shift and an or per pair).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t pack_bf16x2(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 // Accumulators of a wave's 2 x 4 blocks D[neuron][row] seeded with the
 // layer's bias: lane l's value v belongs to neuron 32 mb + 8 (v/4) + 4 (l/32)

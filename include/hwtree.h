@@ -133,6 +133,15 @@ typedef struct {
     int32_t fp64_repriced;    /* HWT_PREC_MIXED: options whose FP32 tree state left FP32's
                                  range (long, weakly mean-reverting bonds) and were priced
                                  again with FP64 states in the same execution */
+    int64_t wide_step_ops;    /* arithmetic operations the wide-tree kernel's stencil steps
+                                 executed, padding slots included: per lane and step 4K+1 in
+                                 the unit-diagonal form (K nodes per lane), 5K in the plain
+                                 form (FP64 FLOP in FP64 mode).  Counted by hwt_plan_sync
+                                 from the window schedule the kernel walks (a small kernel
+                                 after the execution, run only when stats are asked for);
+                                 0 for form 1 */
+    int64_t wide_unit_slots;  /* node slots it stepped in the unit-diagonal form (64 K per step) */
+    int64_t wide_plain_slots; /* ... in the plain form */
 } hwt_stats;
 
 /* Fills *cfg with defaults (auto form, FP64, tuned sizes, work-descending order). */

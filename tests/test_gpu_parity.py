@@ -258,7 +258,8 @@ def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
     plan.execute(dev)
     st1 = plan.sync()
     assert_parity(dev["out"].to_host(), ref, "first execution")
-    keys = ("tasks", "wide_tasks", "node_updates", "wide_node_updates", "seq_node_updates", "unsupported")
+    keys = ("tasks", "wide_tasks", "node_updates", "wide_node_updates", "seq_node_updates", "unsupported",
+            "wide_step_ops", "wide_unit_slots", "wide_plain_slots")
     assert st1["tasks"] == opts.N and st1["wide_tasks"] > 0
     for _ in range(3):
         plan.execute(dev)
@@ -281,6 +282,30 @@ def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
     assert_parity(dev["out"].to_host(), ref, "after the invalid batch")
     assert {k: st[k] for k in keys} == {k: st1[k] for k in keys}
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_step_counts(pkg, precision):
+    """The wide kernel's own count of what its stencil steps executed.  Every
+    window slot it stepped is counted once, in the form it ran in, so the
+    slots cover the active tree region (the work estimate) with padding on
+    top; the operations follow from the forms: 4K+1 per lane and step in the
+    unit-diagonal form (between 4 and 5 per slot), 5 per slot in the plain
+    form.  FP64 steps every window of these trees in the unit form, mixed
+    precision none."""
+    capi = pkg.capi
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=3, n=3000)
+    prec = capi.PREC_FP64 if precision == "fp64" else capi.PREC_MIXED
+    _, st = capi.price(opts, y, capi.config(form=capi.FORM_WAVE, precision=prec), stats=True)
+    unit, plain, ops = st["wide_unit_slots"], st["wide_plain_slots"], st["wide_step_ops"]
+    assert st["wide_tasks"] > 0
+    assert st["wide_node_updates"] <= unit + plain <= 1.6 * st["wide_node_updates"]
+    assert 4 * unit + 5 * plain <= ops <= 5 * (unit + plain)
+    if precision == "fp64":
+        assert plain == 0 and unit > 0
+    elif st["fp64_repriced"] == 0:
+        assert unit == 0 and ops == 5 * plain
 
 
 def test_plan_capacity_enforced(pkg):
