@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "options/sec on 1 MI355X: FP64 tree vs surrogate+fallback at max err 1e-3"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
 FLOP_PER_NODE_UPDATE = 5
 
 
@@ -121,7 +122,25 @@ def reference_sets_leg(pkg, capi, args):
         plan.close()
         ms = e0.elapsed_ms(e1) / reps
         out[name] = {"ms": ms, "options_per_s": opts.N / (ms * 1e-3), "node_updates": st["node_updates"],
-                     "seq_node_updates": st["seq_node_updates"], "tuned_flags": cfg.flags}
+                     "seq_node_updates": st["seq_node_updates"], "tuned_form": cfg.form, "tuned_flags": cfg.flags}
+        # form 1 (one lane per option, tree columns in HBM) on the same batch,
+        # pricing kernel alone: 16 B of column traffic per node update (the
+        # next value read, the new value written)
+        plan = capi.Plan(opts.N, yref.N, capi.config(form=capi.FORM_LANE, flags=capi.FLAG_NO_GRAPH))
+        plan.execute(dev)
+        s1 = plan.sync()
+        kms = []
+        for _ in range(3 if s1["kernel_ms"] < 50 else 0):
+            plan.execute(dev)
+            kms.append(plan.sync()["kernel_ms"])
+        plan.close()
+        if kms:
+            km = float(np.median(kms))
+            gbs = 16.0 * s1["node_updates"] / (km * 1e-3) / 1e9
+            out[name]["form1"] = {"kernel_ms": km, "options_per_s": opts.N / (km * 1e-3),
+                                  "column_gb_per_s": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        else:
+            out[name]["form1"] = {"kernel_ms_first": s1["kernel_ms"], "note": "not repeated (over 50 ms)"}
         for k in list(dev):
             if hasattr(dev[k], "free"):
                 dev[k].free()
@@ -217,6 +236,9 @@ def main():
     ap.add_argument("--concurrent", action="store_true", help="--profile: narrow-tree kernel beside the wide one")
     ap.add_argument("--no-rows", action="store_true", help="--profile: narrow trees one per wavefront")
     ap.add_argument("--fused", action="store_true", help="--profile: one persistent kernel for every tree")
+    ap.add_argument("--profile-lane", default=None, metavar="SET",
+                    help="form 1 alone, without the graph, on one of the reference's 65,536-option sets "
+                         "(e.g. 0_UNIFORM), for a profiler")
     ap.add_argument("--profile-surrogate", action="store_true",
                     help="profiler run of the surrogate: a short training (2 epochs on 262,144 tree-priced "
                          "samples), then warmup + steps inference-with-fallback passes over the 1M-option mix")
@@ -231,6 +253,24 @@ def main():
     pkg = ge.load_package()
     capi, dm = pkg.capi, pkg.devmem
     y = pkg.datasets.synthetic_yield()
+    if args.profile_lane:
+        yref = pkg.formats.read_yield(ROOT / "tests" / "golden" / "yield.in")
+        opts = pkg.datasets.reference_set(args.profile_lane, 65536, seed=5)
+        dev = capi.to_device(opts, yref)
+        plan = capi.Plan(opts.N, yref.N, capi.config(form=capi.FORM_LANE, flags=capi.FLAG_NO_GRAPH))
+        ks = []
+        for i in range(args.warmup + args.steps):
+            plan.execute(dev)
+            st = plan.sync()
+            if i >= args.warmup:
+                ks.append(st["kernel_ms"])
+            print(f"profile lane step {i}: form-1 kernel {st['kernel_ms']:.4f} ms", flush=True)
+        plan.close()
+        km = float(np.mean(ks))
+        print(json.dumps({"profile_lane": args.profile_lane, "options": opts.N, "kernel_ms_mean": km,
+                          "node_updates": st["node_updates"],
+                          "column_gb_per_s": 16.0 * st["node_updates"] / (km * 1e-3) / 1e9}), flush=True)
+        return
     if args.profile_surrogate:
         S = pkg.surrogate
         s, _ = S.fit(y, n_train=262144, epochs=2, seed=11)
