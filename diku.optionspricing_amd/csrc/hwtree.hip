@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -564,7 +565,10 @@ struct hwt_plan {
     bool timed_split = false;   // so were evw0/evw1 and evn0/evn1
     int32_t last_n = 0;         // options bound of the last execution
     bool steps_counted = true;  // k_count_steps has run for the last execution
+    uint64_t serial = 0;        // unique per plan in the process (graphs captured over it key on it)
 };
+
+static std::atomic<uint64_t> g_plan_serial{0};
 
 static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, double *out, hipStream_t s, bool time_kernel)
 {
@@ -697,6 +701,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     *out = nullptr;
     hwt_plan *p = new (std::nothrow) hwt_plan();
     if (!p) return fail(HWT_ENOMEM, "hwt_plan_create: host allocation");
+    p->serial = ++g_plan_serial;
     if (cfg) p->cfg = *cfg;
     else hwt_config_init(&p->cfg);
     if (p->cfg.form < HWT_FORM_AUTO || p->cfg.form > HWT_FORM_WAVE ||
@@ -902,30 +907,32 @@ int hwt_plan_execute(hwt_plan *p, const hwt_options_f64 *dopts, const hwt_yield_
 
 int hwt_internal_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
 
-// In-stream variant for callers that only know the option count on the
-// device (the surrogate's fallback list): arrays of dopts->n (the bound)
-// options, of which the first *dev_count are priced.  Launched directly on
-// `stream`, never through the plan's graph.
-int hwt_plan_execute_counted(hwt_plan *p, const hwt_options_f64 *dopts, const int32_t *dev_count,
-                             const hwt_yield_f64 *dyield, double *dout, void *stream)
+// The pipeline over arrays holding up to dopts->n options, of which the
+// first *dev_count (device memory) are priced, enqueued on `stream` for
+// capture into a caller's graph (the surrogate's fallback list).
+int hwt_internal_plan_enqueue_counted(hwt_plan *p, const hwt_options_f64 *dopts, const int32_t *dev_count,
+                                      const hwt_yield_f64 *dyield, double *dout, void *stream)
 {
-    g_last_error.clear();
     if (!p || !dev_count || !valid_opts_ptrs(dopts) || !valid_yield_ptrs(dyield) || (dopts->n > 0 && !dout))
-        return fail(HWT_EINVAL, "hwt_plan_execute_counted: null or negative-size argument");
-    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "hwt_plan_execute_counted: bound above the plan capacity");
-    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "hwt_plan_execute_counted: yield curve longer than the plan capacity");
+        return fail(HWT_EINVAL, "plan enqueue: null or negative-size argument");
+    if (dopts->n > p->capacity) return fail(HWT_ENOMEM, "plan enqueue: bound above the plan capacity");
+    if (dyield->n > p->ycap) return fail(HWT_ENOMEM, "plan enqueue: yield curve longer than the plan capacity");
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
     DevOpts o = dev_opts(*dopts);
     o.count = dev_count;
-    const DevYield y{dyield->n, dyield->rate, dyield->day};
     p->last_stream = s;
     p->last_n = o.n;
-    p->steps_counted = false;
-    HWT_HIP(hipEventRecord(p->ev0, s));
-    const int rc = launch_pipeline(p, o, y, dout, s, true);
-    if (rc) return rc;
-    HWT_HIP(hipEventRecord(p->ev1, s));
-    return HWT_OK;
+    p->steps_counted = true;  // nothing to count: the caller's graph has no stats
+    return launch_pipeline(p, o, DevYield{dyield->n, dyield->rate, dyield->day}, dout, s, false);
+}
+
+uint64_t hwt_internal_plan_serial(const hwt_plan *p) { return p ? p->serial : 0; }
+
+void hwt_internal_plan_replayed_on(hwt_plan *p, void *stream)
+{
+    if (!p) return;
+    p->last_stream = stream ? (hipStream_t)stream : p->stream;
+    p->steps_counted = true;
 }
 
 int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)

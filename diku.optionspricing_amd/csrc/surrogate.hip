@@ -836,6 +836,10 @@ __global__ void k_fb_scatter(const int32_t *__restrict__ idx, const int32_t *__r
     out[idx[j]] = tmp[j];
 }
 
+// The device's constant-rate wall clock into slot i (a pass's timing marks,
+// which events cannot give inside a graph).
+__global__ void k_stamp(uint64_t *t, int i) { t[i] = __builtin_amdgcn_s_memrealtime(); }
+
 __global__ void k_zero_i32(int32_t *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -889,12 +893,27 @@ struct hwt_surrogate {
     int32_t *bad = nullptr;
     int32_t *fb_idx = nullptr;   // fallback list: out-of-box options, then uncertain in-box ones
     int32_t *inf_rows = nullptr; // in-box options, the rows of the inference kernel
-    int32_t *fb_count = nullptr; // [0] fallback total, [1] of which out of box, [4] inference rows
+    int32_t *fb_count = nullptr; // [0] fallback total, [1] of which out of box, [4] inference rows;
+                                 // [8..13] three u64 wall-clock stamps of a pricing pass
     double *fb_tmp = nullptr;
     Compact comp{};
-    int32_t *h_count = nullptr;  // pinned [2]
+    int32_t *h_count = nullptr;  // pinned copy of fb_count [16]
+    int wall_khz = 100000;       // rate of the device's constant wall clock
     double *h_loss = nullptr;    // pinned [members]
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    // the pricing pass captured as one graph, and the arguments it was
+    // captured with
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct PassKey {
+        hwt_options_f64 o;
+        hwt_yield_f64 y;
+        double *out;
+        const hwt_plan *tree;
+        uint64_t tree_serial;
+        float threshold;
+        int32_t members;
+    } key{};
 };
 
 namespace {
@@ -1063,6 +1082,8 @@ void hwt_surr_destroy(hwt_surrogate *s)
     if (s->h_loss) (void)hipHostFree(s->h_loss);
     for (hipEvent_t e : {s->e0, s->e1, s->e2})
         if (e) (void)hipEventDestroy(e);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -1099,6 +1120,12 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
         return bail(HWT_EHIP, "hipStreamCreate failed");
     for (hipEvent_t *e : {&s->e0, &s->e1, &s->e2})
         if (hipEventCreate(e) != hipSuccess) return bail(HWT_EHIP, "hipEventCreate failed");
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) == hipSuccess && khz > 0)
+            s->wall_khz = khz;
+        (void)hipGetLastError();
+    }
     const size_t M = s->members, B = s->B, T = std::max(train_capacity, 1), I = std::max(infer_capacity, 1);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
@@ -1114,7 +1141,7 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
     const size_t oy = carve(M * B * 4), ogw = carve(M * 3 * DW_SPLIT * LAYER_W * 4), ogs = carve(M * PSTRIDE * 4);
     const size_t ohp = carve(M * (B / HEAD_ROWS) * (HID + 1) * 4), oloss = carve(M * 8), og = carve(M * PSTRIDE * 4);
     const size_t oX = carve(T * NIN * 4), otg = carve(T * 4), operm = carve(M * T * 4), orows = carve(B * 4);
-    const size_t obad = carve(4), ofbi = carve(I * 4), ofbc = carve(8 * 4), ofbt = carve(I * 8);
+    const size_t obad = carve(4), ofbi = carve(I * 4), ofbc = carve(16 * 4), ofbt = carve(I * 8);
     const size_t oinr = carve(I * 4);
     const size_t ocs = carve(I * 8), ocm = carve(I * 8), ocl = carve(I * 8), oca = carve(I * 8), ocg = carve(I * 8);
     const size_t ocu = carve(I * 2), ocst = carve(I * 2), octy = carve(I);
@@ -1154,7 +1181,7 @@ int hwt_surr_create(hwt_surrogate **out, int32_t train_capacity, int32_t infer_c
     s->fb_tmp = (double *)(b + ofbt);
     s->comp = Compact{(double *)(b + ocs), (double *)(b + ocm), (double *)(b + ocl), (double *)(b + oca),
                       (double *)(b + ocg), (uint16_t *)(b + ocu), (uint16_t *)(b + ocst), (int8_t *)(b + octy)};
-    if (hipHostMalloc((void **)&s->h_count, 8 * sizeof(int32_t), 0) != hipSuccess ||
+    if (hipHostMalloc((void **)&s->h_count, 16 * sizeof(int32_t), 0) != hipSuccess ||
         hipHostMalloc((void **)&s->h_loss, M * sizeof(double), 0) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     // zero everything once (gradient partials of unused rows stay defined)
@@ -1338,6 +1365,54 @@ int hwt_surr_predict(hwt_surrogate *s, const hwt_options_f64 *o, const hwt_yield
     return HWT_OK;
 }
 
+// One pricing pass, enqueued on st (for capture): the box split, the ensemble
+// on the in-box rows, then the whole fallback list on the FP64 tree in one
+// pass and its prices back to input order.  e1 marks the end of inference.
+static int enqueue_pass(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, const hwt_yield_f64 *y,
+                        double *out, hipStream_t st)
+{
+    uint64_t *stamps = reinterpret_cast<uint64_t *>(s->fb_count + 8);
+    k_stamp<<<1, 1, 0, st>>>(stamps, 0);
+    k_zero_i32<<<1, 32, 0, st>>>(s->fb_count, 8);
+    if (o->n == 0) {
+        k_stamp<<<1, 1, 0, st>>>(stamps, 1);
+        k_stamp<<<1, 1, 0, st>>>(stamps, 2);
+        SURR_HIP(hipGetLastError());
+        return HWT_OK;
+    }
+    const int blocks = (o->n + 255) / 256;
+    // split: out-of-box options to the fallback list, in-box ones to the
+    // inference rows (running the tree on the out-of-box part beside the
+    // inference was measured slower: both compete for the same CUs)
+    k_box<<<(o->n + BOX_CHUNK - 1) / BOX_CHUNK, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->inf_rows,
+                                                            s->fb_count + 4);
+    // the ensemble on the in-box rows; options it is unsure of join the list
+    InferArgs g{};
+    g.o = sopts(*o);
+    g.y = SYield{y->n, y->rate, y->day};
+    g.net = s->net;
+    g.members = s->members;
+    g.threshold = s->cfg.spread_threshold;
+    g.out = out;
+    g.fb_idx = s->fb_idx;
+    g.fb_count = s->fb_count;
+    g.rows = s->inf_rows;
+    g.row_count = s->fb_count + 4;
+    k_infer<<<(o->n + INF_ROWS - 1) / INF_ROWS, 256, 0, st>>>(g);
+    k_stamp<<<1, 1, 0, st>>>(stamps, 1);
+    SURR_HIP(hipGetLastError());
+    // the whole list on the FP64 tree in one pass
+    k_fb_gather<<<blocks, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->comp);
+    const hwt_options_f64 co{o->n,          s->comp.strike, s->comp.maturity, s->comp.length, s->comp.tu,
+                             s->comp.steps, s->comp.a,      s->comp.sigma,    s->comp.type};
+    const int rc = hwt_internal_plan_enqueue_counted(tree, &co, s->fb_count, y, s->fb_tmp, st);
+    if (rc) return rc;
+    k_fb_scatter<<<blocks, 256, 0, st>>>(s->fb_idx, s->fb_count, s->fb_tmp, out);
+    k_stamp<<<1, 1, 0, st>>>(stamps, 2);
+    SURR_HIP(hipGetLastError());
+    return HWT_OK;
+}
+
 int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, const hwt_yield_f64 *y, double *out,
                    hwt_surr_stats *stats)
 {
@@ -1352,51 +1427,52 @@ int hwt_surr_price(hwt_surrogate *s, hwt_plan *tree, const hwt_options_f64 *o, c
                                 "set spread_threshold to +inf to take every in-box price from the network");
     DeviceGuardS dg(s->device);
     hipStream_t st = s->stream;
-    SURR_HIP(hipEventRecord(s->e0, st));
-    k_zero_i32<<<1, 32, 0, st>>>(s->fb_count, 8);
-    if (o->n > 0) {
-        const int blocks = (o->n + 255) / 256;
-        // split: out-of-box options to the fallback list, in-box ones to the
-        // inference rows (running the tree on the out-of-box part beside the
-        // inference was measured slower: both compete for the same CUs)
-        k_box<<<(o->n + BOX_CHUNK - 1) / BOX_CHUNK, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->inf_rows,
-                                                                s->fb_count + 4);
-        // the ensemble on the in-box rows; options it is unsure of join the list
-        InferArgs g{};
-        g.o = sopts(*o);
-        g.y = SYield{y->n, y->rate, y->day};
-        g.net = s->net;
-        g.members = s->members;
-        g.threshold = s->cfg.spread_threshold;
-        g.out = out;
-        g.fb_idx = s->fb_idx;
-        g.fb_count = s->fb_count;
-        g.rows = s->inf_rows;
-        g.row_count = s->fb_count + 4;
-        k_infer<<<(o->n + INF_ROWS - 1) / INF_ROWS, 256, 0, st>>>(g);
-        SURR_HIP(hipGetLastError());
-        SURR_HIP(hipEventRecord(s->e1, st));
-        // the whole list on the FP64 tree in one pass
-        k_fb_gather<<<blocks, 256, 0, st>>>(sopts(*o), s->fb_idx, s->fb_count, s->comp);
-        const hwt_options_f64 co{o->n,          s->comp.strike, s->comp.maturity, s->comp.length, s->comp.tu,
-                                 s->comp.steps, s->comp.a,      s->comp.sigma,    s->comp.type};
-        const int rc = hwt_plan_execute_counted(tree, &co, s->fb_count, y, s->fb_tmp, st);
-        if (rc) return rc;
-        k_fb_scatter<<<blocks, 256, 0, st>>>(s->fb_idx, s->fb_count, s->fb_tmp, out);
-        SURR_HIP(hipGetLastError());
-    } else {
-        SURR_HIP(hipEventRecord(s->e1, st));
+    // The whole pass -- box split, ensemble inference and gate, and the FP64
+    // tree on the fallback list -- is one graph, captured on the first call
+    // and again when its arguments change, then replayed.
+    hwt_surrogate::PassKey key;
+    std::memset(&key, 0, sizeof(key));  // padding too: the key is compared bytewise
+    key.o = *o;
+    key.y = *y;
+    key.out = out;
+    key.tree = tree;
+    key.tree_serial = hwt_internal_plan_serial(tree);
+    key.threshold = s->cfg.spread_threshold;
+    key.members = s->members;
+    if (!s->exec || std::memcmp(&key, &s->key, sizeof(key)) != 0) {
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
+        s->exec = nullptr;
+        s->graph = nullptr;
+        (void)hipGetLastError();
+        SURR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_pass(s, tree, o, y, out, st);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(HWT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        s->graph = g;
+        SURR_HIP(hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0));
+        s->key = key;
     }
+    SURR_HIP(hipEventRecord(s->e0, st));
+    SURR_HIP(hipGraphLaunch(s->exec, st));
+    if (o->n > 0) hwt_internal_plan_replayed_on(tree, st);
     SURR_HIP(hipEventRecord(s->e2, st));
-    SURR_HIP(hipMemcpyAsync(s->h_count, s->fb_count, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SURR_HIP(hipMemcpyAsync(s->h_count, s->fb_count, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SURR_HIP(hipStreamSynchronize(st));
     const int trc = o->n > 0 ? hwt_plan_sync(tree, nullptr) : HWT_OK;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, s->e0, s->e2) == hipSuccess) stats->ms = ms;
-        if (hipEventElapsedTime(&ms, s->e0, s->e1) == hipSuccess) stats->surrogate_ms = ms;
-        if (hipEventElapsedTime(&ms, s->e1, s->e2) == hipSuccess) stats->tree_ms = ms;
+        uint64_t t[3];
+        std::memcpy(t, s->h_count + 8, sizeof(t));
+        stats->surrogate_ms = (double)(t[1] - t[0]) / s->wall_khz;
+        stats->tree_ms = (double)(t[2] - t[1]) / s->wall_khz;
         stats->fallback = s->h_count[0];
         stats->out_of_box = s->h_count[1];
         stats->steps = s->step;

@@ -75,8 +75,10 @@ typedef struct {
     int64_t steps;            /* optimiser steps taken so far */
     int32_t fallback;         /* options re-priced by the tree (hwt_surr_price) */
     int32_t out_of_box;       /* ... of which outside the training box */
-    double  surrogate_ms;     /* box split + features + ensemble inference + gate (hwt_surr_price) */
-    double  tree_ms;          /* fallback tree pricing (hwt_surr_price) */
+    double  surrogate_ms;     /* box split + features + ensemble inference + gate (hwt_surr_price;
+                                 read from the device's wall clock at the pass's marks, since the
+                                 pass runs as one graph) */
+    double  tree_ms;          /* fallback tree pricing (hwt_surr_price; the same way) */
     int32_t reserved[4];
 } hwt_surr_stats;
 

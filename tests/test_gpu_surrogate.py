@@ -158,6 +158,44 @@ def test_out_of_box_always_falls_back(pkg, setup):
     plan.close()
 
 
+def test_pass_graph_replays_and_recaptures(pkg, setup):
+    """A pricing pass is one graph: captured on the first call, replayed
+    after, captured again when an argument changes (output buffer, tree
+    plan).  Every replay, with the tree plan's own executions and host
+    synchronisations between them, gives the first call's prices and
+    fallback counts bit for bit."""
+    capi, y, s = pkg.capi, setup["y"], setup["s"]
+    opts = pkg.datasets.box_mix(51, 3000)
+    dev = capi.to_device(opts, y)
+    other = pkg.datasets.box(53, 500)
+    dother = capi.to_device(other, y)
+    plan = capi.Plan(4096, y.N)
+    plan.execute(dother)
+    plan.sync()
+    ref_other = dother["out"].to_host()
+    st1 = s.price(plan, dev)
+    got1 = dev["out"].to_host()
+    assert st1["fallback"] >= st1["out_of_box"] > 0 and np.all(np.isfinite(got1))
+    for _ in range(3):
+        dother["out"].fill_zero()
+        plan.execute(dother)  # the tree plan's own graph between the passes
+        plan.sync()
+        np.testing.assert_array_equal(dother["out"].to_host(), ref_other)
+        dev["out"].fill_zero()
+        st = s.price(plan, dev)
+        np.testing.assert_array_equal(dev["out"].to_host(), got1)
+        assert (st["fallback"], st["out_of_box"]) == (st1["fallback"], st1["out_of_box"])
+    out2 = pkg.devmem.DeviceArray.empty((opts.N,), "float64")
+    s.price(plan, dev, out=out2)  # another output buffer: captured again
+    np.testing.assert_array_equal(out2.to_host(), got1)
+    plan2 = capi.Plan(4096, y.N, capi.config(flags=capi.FLAG_NO_GRAPH))
+    dev["out"].fill_zero()
+    s.price(plan2, dev)  # another tree plan: captured again
+    np.testing.assert_array_equal(dev["out"].to_host(), got1)
+    plan2.close()
+    plan.close()
+
+
 def test_huge_threshold_takes_every_in_box_price_from_the_network(pkg, setup):
     y = setup["y"]
     o = pkg.datasets.box(43, 2000)
