@@ -71,7 +71,7 @@ enum Tally {
     T_NCOUNT = 14,  // end of the execution counts (slots 16-23 are k_setup's accumulators)
     T_QUEUE_R = 15, // next chunk of row groups (fused kernel)
     // accumulated by k_setup's blocks, then moved into the slots above and
-    // cleared by k_scan
+    // cleared by k_order's writer block
     A_WORK = 16,    // int64
     A_SEQWORK = 18, // int64
     A_WORKW = 20,   // int64
@@ -82,7 +82,6 @@ enum Tally {
     T_NROW = 26,    // end of the row-class tasks (start of the flat class)
     T_QUEUE_F = 27, // next chunk of flat groups (fused kernel)
     T_NFLATG = 28,  // flat groups in all
-    T_NCOUNT_ALL = 29,  // [0, T_NCOUNT_ALL) is copied to the host after an execution
     // first task and first group of each flat part (one width each, widest
     // first, as the scan lays them out), then the end of the last part
     T_FLAT_START = 32,  // FLAT_PARTS + 1 entries
@@ -92,8 +91,22 @@ enum Tally {
     T_XFLOP = 64,
     T_XUNIT = 66,
     T_XPLAIN = 68,
-    T_ALLOC = 72
+    T_HOST = 72,  // [0, T_HOST) is copied to the host by hwt_plan_sync
+    // the bucket histogram (k_setup's atomics) and the scatter cursors
+    // (k_order's), cleared by the first pricing kernel of each execution
+    T_HIST = 72,
+    T_CURSOR = T_HIST + NBUCKET,
+    T_ALLOC = T_CURSOR + NBUCKET
 };
+
+// The bucket histogram and scatter cursors (in the tally) back to zero for
+// the next execution: their last readers, k_order's blocks, are done once a
+// pricing kernel runs.
+__device__ __forceinline__ void clear_bucket_counts(int32_t *tally)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * NBUCKET; i += gridDim.x * blockDim.x)
+        tally[T_HIST + i] = 0;
+}
 
 // Pricing kernels (hwt_pricers.h).  All are persistent: one single-wave
 // workgroup per resident slot, tasks from the ordered task list.

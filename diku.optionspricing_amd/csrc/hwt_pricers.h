@@ -195,6 +195,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
     int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch, int32_t *__restrict__ redo)
 {
+    clear_bucket_counts(tally);
     const int nlarge = uni(tally[T_NLARGE]);
     price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch, redo);
 }
@@ -350,6 +351,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     const OptConst *__restrict__ cst, const double *__restrict__ edge, const int32_t *__restrict__ order,
     int32_t *__restrict__ tally, double *__restrict__ out, double *__restrict__ scratch, int32_t *__restrict__ redo)
 {
+    clear_bucket_counts(tally);
     const int nlarge = uni(tally[T_NLARGE]), nmid = uni(tally[T_NMID]), nrow = uni(tally[T_NROW]);
     price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch, redo);
     price_tasks<T, KSMALL, true, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr, redo);

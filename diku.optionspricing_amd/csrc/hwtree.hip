@@ -4,11 +4,13 @@
 //   k_setup    one thread per option: tree constants (the reference's
 //              OptionConstants), P(e dt) and P(n dt) from the yield curve, edge
 //              factors, work estimate; trivial options are priced right here
-//   k_scan     bucket offsets, largest work first (longest-processing-time
-//              order); it also clears every counter for the next execution,
-//              so no clearing kernel or memset node runs per execution
-//              (k_reset only runs for an empty batch, which has no k_scan)
-//   k_scatter  option indices into the task list
+//   k_order    bucket offsets, largest work first (longest-processing-time
+//              order; each block scans the histogram itself), and the option
+//              indices into the task list; it clears the queue heads and the
+//              setup accumulators for the next execution, and the first
+//              pricing kernel clears the histogram and cursors, so no clearing
+//              kernel or memset node runs per execution (k_reset only runs for
+//              an empty batch, which has no k_setup)
 //   k_price_wide / k_price_narrow  persistent kernels, launched side by side:
 //              each wavefront pulls options off its part of the task list and
 //              prices them with the on-chip tree column (hwt_wave.h); trees up to
@@ -85,13 +87,9 @@ __device__ __forceinline__ int part_group(int part)
 constexpr int NPARTS = ROW_MAXK + FLAT_PARTS;
 
 // Every counter and tally to zero: the empty batch's whole pipeline.
-__global__ void k_reset(int32_t *tally, int32_t *hist, int32_t *cursor)
+__global__ void k_reset(int32_t *tally)
 {
-    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < NBUCKET; i += blockDim.x * gridDim.x) {
-        if (i < T_ALLOC) tally[i] = 0;
-        hist[i] = 0;
-        cursor[i] = 0;
-    }
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < T_ALLOC; i += blockDim.x * gridDim.x) tally[i] = 0;
 }
 
 // One thread per option; launched with SETUP_BLOCK threads per block.  The
@@ -114,18 +112,20 @@ __device__ __forceinline__ int scan_bucket(int p, int asc)
     return NBUCKET - 1 - p;
 }
 
-// One block of SETUP_BLOCK threads, after k_setup: exclusive offsets of the
-// buckets in scan order, the class boundaries and counts into the
-// host-visible tally, and every counter cleared for the next execution
-// (histogram, scatter cursors, queue heads, accumulators).  Each part of the
+// Run by every block of k_order (SETUP_BLOCK threads), after k_setup: the
+// exclusive offsets of the buckets in scan order, into soffs (LDS).  The
+// writer block also puts the pads into the task list, the class boundaries
+// and counts into the host-visible tally, and clears the queue heads and the
+// setup accumulators for the next execution; the histogram and the scatter
+// cursors, which every block reads, are cleared by the first pricing kernel.  Each part of the
 // row class (one K) is padded to a multiple of four tasks and each part of the
 // flat class (one width w) to a multiple of 64 / w, with idle slots (order
 // -1), so every group a wave takes shares one K or one width and an option's
 // arithmetic never depends on which options share its wave.  The flat parts'
 // first tasks and first group numbers go to the tally for the flat kernel.
-__global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist, int32_t *__restrict__ offs,
-                                                      int32_t *__restrict__ cursor, int32_t *__restrict__ order,
-                                                      int32_t *__restrict__ tally, int32_t asc, int32_t classes)
+__device__ __forceinline__ void scan_offsets(const int32_t *__restrict__ hist, int32_t *soffs,
+                                             int32_t *__restrict__ order, int32_t *__restrict__ tally, int32_t asc,
+                                             int32_t classes, bool writer)
 {
     const int tid = threadIdx.x;
     constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NR = B_MID - B_ROW;
@@ -142,11 +142,8 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
     if (tid == 0) npad = 0;
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < SCAN_PER; ++q) {
-        hist[idx[q]] = 0;
-        cursor[idx[q]] = 0;
+    for (int q = 0; q < SCAN_PER; ++q)
         if (idx[q] < B_MID && cnt[q]) atomicAdd(&ptot[part_of(idx[q])], cnt[q]);
-    }
     __syncthreads();
     // the lowest bucket of each part comes last in the scan and takes the pad
     int v[SCAN_PER], pad[SCAN_PER], sum = 0, pads = 0;
@@ -173,16 +170,18 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_scan(int32_t *__restrict__ hist
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
         const int p = tid * SCAN_PER + q;
-        offs[idx[q]] = run;
+        soffs[idx[q]] = run;
         run += v[q];
-        for (int j = 0; j < pad[q]; ++j) order[run - pad[q] + j] = -1;
-        if (p == NW - 1) tally[T_NLARGE] = run;
-        if (p == NW + NM - 1) tally[T_NMID] = run;
-        if (p == NW + NM + NR - 1) tally[T_NROW] = run;
-        if (p == NBUCKET - 1) tally[T_NTASK] = run;
+        if (writer) {
+            for (int j = 0; j < pad[q]; ++j) order[run - pad[q] + j] = -1;
+            if (p == NW - 1) tally[T_NLARGE] = run;
+            if (p == NW + NM - 1) tally[T_NMID] = run;
+            if (p == NW + NM + NR - 1) tally[T_NROW] = run;
+            if (p == NBUCKET - 1) tally[T_NTASK] = run;
+        }
     }
     __syncthreads();  // orders the boundary writes above before thread 0 reads them
-    if (tid == 0) {
+    if (writer && tid == 0) {
         // flat parts in task order, widest first: first task and first group
         int start = tally[T_NROW], group = 0;
         for (int j = 0; j < FLAT_PARTS; ++j) {
@@ -356,28 +355,33 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     }
 }
 
-// Option indices into the task list, bucket by bucket.  Ranks inside a block
+// After k_setup: the bucket offsets (every block scans the 8 KB histogram
+// itself, scan_offsets; block 0 also writes the pads and the tally), then the
+// option indices into the task list, bucket by bucket.  Ranks inside a block
 // come from LDS atomics; one global atomic per (block, bucket) reserves the
-// block's slots.  Order inside a bucket is arbitrary (options are independent).
-__global__ __launch_bounds__(SETUP_BLOCK) void k_scatter(int32_t n, const int32_t *__restrict__ count,
-                                                         const int32_t *__restrict__ bucket,
-                                                         const int32_t *__restrict__ offs,
-                                                         int32_t *__restrict__ cursor, int32_t *__restrict__ order)
+// block's slots.  Order inside a bucket is arbitrary (options are
+// independent).  One kernel instead of a scan kernel and a scatter kernel
+// saves a kernel boundary per execution.
+__global__ __launch_bounds__(SETUP_BLOCK) void k_order(int32_t n, const int32_t *__restrict__ count,
+                                                       const int32_t *__restrict__ bucket,
+                                                       const int32_t *__restrict__ hist, int32_t *__restrict__ cursor,
+                                                       int32_t *__restrict__ order, int32_t *__restrict__ tally,
+                                                       int32_t asc, int32_t classes)
 {
-    __shared__ int32_t lcount[NBUCKET], lbase[NBUCKET];
+    __shared__ int32_t soffs[NBUCKET], lcount[NBUCKET], lbase[NBUCKET];
     const int tid = threadIdx.x;
-    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lcount[k] = 0;
-    __syncthreads();
     const int i = tid + blockIdx.x * SETUP_BLOCK;
     if (count) n = min(*count, n);
-    const int b = (i < n) ? bucket[i] : -1;
-    const int ob = (b >= 0) ? offs[b] : 0;  // requested before the barriers, so it arrives meanwhile
+    const int b = (i < n) ? bucket[i] : -1;  // requested first, so it arrives during the scan
+    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lcount[k] = 0;
+    scan_offsets(hist, soffs, order, tally, asc, classes, blockIdx.x == 0);
+    __syncthreads();
     const int r = (b >= 0) ? atomicAdd(&lcount[b], 1) : 0;
     __syncthreads();
     for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
         if (lcount[k]) lbase[k] = atomicAdd(&cursor[k], lcount[k]);
     __syncthreads();
-    if (b >= 0) order[ob + lbase[b] + r] = i;
+    if (b >= 0) order[soffs[b] + lbase[b] + r] = i;
 }
 
 // What the wide-tree kernel's stencil steps executed in the last execution,
@@ -419,9 +423,10 @@ __global__ __launch_bounds__(256) void k_count_steps(const OptConst *__restrict_
 // Form 1: one lane per option over every scheduled task (tasks ordered by
 // decreasing work, so the lanes of a wave carry similar trees).
 __global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__restrict__ edge,
-                             const int32_t *__restrict__ order, const int32_t *__restrict__ tally,
+                             const int32_t *__restrict__ order, int32_t *__restrict__ tally,
                              double *__restrict__ out, double *__restrict__ cols, int32_t colw)
 {
+    clear_bucket_counts(tally);
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= tally[T_NTASK]) return;
     const int o = order[gid];
@@ -549,7 +554,7 @@ struct hwt_plan {
     size_t ws_bytes = 0;
     OptConst *cst = nullptr;
     double *edge = nullptr;
-    int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *offs = nullptr, *tally = nullptr;
+    int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *tally = nullptr;
     int32_t *h_tally = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     hipEvent_t evw0 = nullptr, evw1 = nullptr, evn0 = nullptr, evn1 = nullptr;
@@ -580,9 +585,9 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
     p->timed_split = p->timed_kernel && p->cfg.form != HWT_FORM_LANE;
     const int sort = p->cfg.sort ? p->cfg.sort : 'L';
     const int asc = (sort == 'w' || sort == 'h') ? 1 : 0;
-    // an empty batch launches no k_setup and no k_scan: the tallies it
+    // an empty batch launches no k_setup and no k_order: the tallies it
     // reports (all zero) come from a clearing kernel instead
-    if (n == 0) k_reset<<<1, 1024, 0, s>>>(p->tally, p->hist, p->cursor);
+    if (n == 0) k_reset<<<1, 1024, 0, s>>>(p->tally);
     if (n > 0) {
         const int blocks = (n + SETUP_BLOCK - 1) / SETUP_BLOCK;
         const double unit_dmin = p->cfg.precision == HWT_PREC_MIXED ? UnitRange<float>::dmin : UnitRange<double>::dmin;
@@ -595,8 +600,8 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         const int max_width = p->cfg.form == HWT_FORM_LANE ? p->lane_width : WAVE_MAX_WIDTH;
         k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, max_width, unit_dmin, classes, p->cst, p->edge,
                                                p->bucket, p->hist, p->tally, out);
-        k_scan<<<1, SETUP_BLOCK, 0, s>>>(p->hist, p->offs, p->cursor, p->order, p->tally, asc, classes);
-        k_scatter<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->offs, p->cursor, p->order);
+        k_order<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->hist, p->cursor, p->order, p->tally, asc,
+                                               classes);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
             const int lb = p->lane_block;
@@ -777,10 +782,7 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     const size_t o_edge = carve(cap * EDGE_TAB * sizeof(double));
     const size_t o_bucket = carve(cap * sizeof(int32_t));
     const size_t o_order = carve((cap + MAX_PADS) * sizeof(int32_t));  // + row and flat pads
-    const size_t o_hist = carve(NBUCKET * sizeof(int32_t));
-    const size_t o_cursor = carve(NBUCKET * sizeof(int32_t));
-    const size_t o_offs = carve(NBUCKET * sizeof(int32_t));
-    const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));
+    const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));  // with the histogram and scatter cursors
     const bool lane = p->cfg.form == HWT_FORM_LANE;
     // the wide-tree kernels' parked forward states (form 2 only)
     const size_t o_scw = carve(lane ? 0 : (size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
@@ -800,22 +802,19 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     p->edge = (double *)(base + o_edge);
     p->bucket = (int32_t *)(base + o_bucket);
     p->order = (int32_t *)(base + o_order);
-    p->hist = (int32_t *)(base + o_hist);
-    p->cursor = (int32_t *)(base + o_cursor);
-    p->offs = (int32_t *)(base + o_offs);
     p->tally = (int32_t *)(base + o_tally);
+    p->hist = p->tally + T_HIST;
+    p->cursor = p->tally + T_CURSOR;
     p->scratch_wide = (double *)(base + o_scw);
     p->redo = p->cfg.precision == HWT_PREC_MIXED ? (int32_t *)(base + o_redo) : nullptr;
     p->cols = lane ? (double *)(base + o_cols) : nullptr;
     // counters start clear; from then on every execution leaves them clear
-    if (hipMemsetAsync(p->hist, 0, NBUCKET * sizeof(int32_t), p->stream) != hipSuccess ||
-        hipMemsetAsync(p->cursor, 0, NBUCKET * sizeof(int32_t), p->stream) != hipSuccess ||
-        hipMemsetAsync(p->tally, 0, T_ALLOC * sizeof(int32_t), p->stream) != hipSuccess ||
+    if (hipMemsetAsync(p->tally, 0, T_ALLOC * sizeof(int32_t), p->stream) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess)
         return bail(HWT_EHIP, "clearing the plan's counters failed");
-    if (hipHostMalloc((void **)&p->h_tally, T_ALLOC * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&p->h_tally, T_HOST * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
-    std::memset(p->h_tally, 0, T_ALLOC * sizeof(int32_t));  // a plan never executed syncs clean
+    std::memset(p->h_tally, 0, T_HOST * sizeof(int32_t));  // a plan never executed syncs clean
     if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
         hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
         hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||
@@ -952,7 +951,7 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
     // the tallies are fetched here, once, rather than by a copy queued behind
     // every execution: back-to-back executions then run without a transfer
     // between them
-    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_ALLOC * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HWT_HIP(hipMemcpyAsync(p->h_tally, p->tally, T_HOST * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HWT_HIP(hipStreamSynchronize(s));
     const int32_t *t = p->h_tally;
     if (stats) {
@@ -1389,8 +1388,7 @@ int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, c
 // caller's inputs and output in this one static device array, which each
 // device holds from the module's load until the process ends.  Layout (bytes,
 // each part 256-aligned): OptConst[N], edge tables[N][36], bucket[N],
-// order[N + row pads], histogram, scatter cursors and offsets (1,024 each),
-// counters (32), and the wide-tree kernel's parked forward states (one
+// order[N + row pads], counters with the histogram and scatter cursors, and the wide-tree kernel's parked forward states (one
 // 64 x MAXK double slot per resident wave, at most NOALLOC_WIDE_SLOTS).
 constexpr int NOALLOC_MAX_OPTIONS = HWT_NOALLOC_MAX_OPTIONS;
 constexpr int NOALLOC_WIDE_SLOTS = 1024;
@@ -1399,10 +1397,7 @@ constexpr size_t NOALLOC_O_CST = 0;
 constexpr size_t NOALLOC_O_EDGE = NOALLOC_O_CST + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * sizeof(OptConst));
 constexpr size_t NOALLOC_O_BUCKET = NOALLOC_O_EDGE + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * EDGE_TAB * 8);
 constexpr size_t NOALLOC_O_ORDER = NOALLOC_O_BUCKET + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * 4);
-constexpr size_t NOALLOC_O_HIST = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + MAX_PADS) * 4);
-constexpr size_t NOALLOC_O_CURSOR = NOALLOC_O_HIST + noalloc_part(NBUCKET * 4);
-constexpr size_t NOALLOC_O_OFFS = NOALLOC_O_CURSOR + noalloc_part(NBUCKET * 4);
-constexpr size_t NOALLOC_O_TALLY = NOALLOC_O_OFFS + noalloc_part(NBUCKET * 4);
+constexpr size_t NOALLOC_O_TALLY = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + MAX_PADS) * 4);
 constexpr size_t NOALLOC_O_WIDE = NOALLOC_O_TALLY + noalloc_part(T_ALLOC * 4);
 constexpr size_t NOALLOC_BYTES = NOALLOC_O_WIDE + (size_t)NOALLOC_WIDE_SLOTS * WAVE * MAXK * 8;
 
@@ -1479,16 +1474,15 @@ int hwt_price_device_noalloc(const hwt_options_f64 *dopts, const hwt_yield_f64 *
     v.edge = (double *)(d.ptr + NOALLOC_O_EDGE);
     v.bucket = (int32_t *)(d.ptr + NOALLOC_O_BUCKET);
     v.order = (int32_t *)(d.ptr + NOALLOC_O_ORDER);
-    v.hist = (int32_t *)(d.ptr + NOALLOC_O_HIST);
-    v.cursor = (int32_t *)(d.ptr + NOALLOC_O_CURSOR);
-    v.offs = (int32_t *)(d.ptr + NOALLOC_O_OFFS);
     v.tally = (int32_t *)(d.ptr + NOALLOC_O_TALLY);
+    v.hist = v.tally + T_HIST;
+    v.cursor = v.tally + T_CURSOR;
     v.scratch_wide = (double *)(d.ptr + NOALLOC_O_WIDE);
     hipStream_t s = (hipStream_t)stream;
     // the array keeps the last call's contents: counters start from zero by a
     // fill kernel (no memset node, so a caller may capture the call in a graph)
     (void)hipGetLastError();
-    k_reset<<<1, 1024, 0, s>>>(v.tally, v.hist, v.cursor);
+    k_reset<<<1, 1024, 0, s>>>(v.tally);
     HWT_HIP(hipGetLastError());
     return launch_pipeline(&v, dev_opts(*dopts), DevYield{dyield->n, dyield->rate, dyield->day}, dout, s, false);
 }
