@@ -376,38 +376,13 @@ __device__ __forceinline__ void init_window(T (&z)[K], const TreeShape &t, int b
 // stays in range (unit_form_ok); the plain stencil prices the rest.
 //
 // The coefficients are formed one node at a time and transformed on the
-// way (the closed forms, with edge nodes read from the LDS table at a
-// per-lane index), so no plain coefficient array is ever live beside the
+// way (the closed forms, advanced incrementally; in windows that reach an
+// edge, edge nodes are read from the LDS table at a per-lane index and
+// padding nodes get zero), so no plain coefficient array is ever live beside the
 // transformed ones: the set-up needs no more registers than the steps, and
 // the form fits every window size at two waves per SIMD.  Undoing D after the
 // steps needs the scales again: each lane parks them in its own node slots of
 // the LDS state buffer, which the steps leave untouched.  FP64 only.
-
-// Three-point coefficients of node k (closed forms; edge nodes from the
-// table; zero past the top edge).  Eprev, Ek, Enext are the discounts of nodes
-// k-1, k, k+1, already zero outside the tree.
-template <bool FWD>
-__device__ __forceinline__ void node_coefs(const TreeShape &t, int k, double Eprev, double Ek, double Enext,
-                                           const lds_f64 *tab, double &cl, double &cm, double &cr)
-{
-    const int j = k - t.jmax;
-    if (FWD) {
-        cl = pu_std(j - 1, t.M) * Eprev;
-        cm = pm_std(j, t.M) * Ek;
-        cr = pd_std(j + 1, t.M) * Enext;
-    } else {
-        cl = pd_std(j, t.M) * Ek;
-        cm = pm_std(j, t.M) * Ek;
-        cr = pu_std(j, t.M) * Ek;
-    }
-    if (k >= t.w) cl = cm = cr = 0.0;
-    if (k < 3 || (k >= t.w - 3 && k < t.w)) {
-        const int e = 3 * edge_slot_of(k, t.w);
-        cl = tab[e];
-        cm = tab[e + 1];
-        cr = tab[e + 2];
-    }
-}
 
 // Transformed coefficients of this lane's nodes, and z -> D^{-1} z: the chain
 // d_s = c_s d_{s-1} (c_s = cl_s on tree nodes, 1 on padding) is built going
@@ -418,12 +393,27 @@ __device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, 
 {
     const lds_f64 *tab = etab + (FWD ? 0 : 18);
     const int k0 = base + lane * K;
+    // a window strictly inside the tree (every growth and shrink window) has
+    // no edge or padding node: its coefficients are the closed forms alone
+    const bool inner = base >= 4 && base + WAVE * K <= t.w - 4;
     const double q = exp(-t.dr * t.dt);
     auto inside = [&](int k) { return k >= 0 && k < t.w; };
     double E = node_disc(k0 - 1, t.jmax, t.dr, t.dt);  // discount of node k0-1+m at step m
     double Eprev = inside(k0 - 1) ? E : 0.0;
     E *= q;
     double Ek = inside(k0) ? E : 0.0;
+    // the standard probabilities from a_k = (k - jmax) M, advanced by one
+    // addition per node: pu, pm, pd = 1/6 + (a^2 + a)/2, 2/3 - a^2,
+    // 1/6 + (a^2 - a)/2 (forward: the outgoing weights of the sources k-1, k,
+    // k+1 of target k, rolled along)
+    double a = (double)(k0 - 1 - t.jmax) * t.M;
+    double a2 = a * a, pu_prev = 0.0, pm_cur = 0.0;
+    if (FWD) {
+        pu_prev = fma(0.5, a2 + a, 1.0 / 6.0) * Eprev;
+        a += t.M;
+        a2 = a * a;
+        pm_cur = (2.0 / 3.0 - a2) * Ek;
+    }
     double c[K];  // the chain factors, kept for the walk down
     double cl, cm, cr, cr_prev = 0.0, d = 1.0;
 #pragma unroll
@@ -431,7 +421,31 @@ __device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, 
         const int k = k0 + s;
         E *= q;
         const double Enext = inside(k + 1) ? E : 0.0;
-        node_coefs<FWD>(t, k, Eprev, Ek, Enext, tab, cl, cm, cr);
+        if (FWD) {
+            const double pu_cur = fma(0.5, a2 + a, 1.0 / 6.0) * Ek;
+            a += t.M;
+            a2 = a * a;
+            cl = pu_prev;
+            cm = pm_cur;
+            cr = fma(0.5, a2 - a, 1.0 / 6.0) * Enext;
+            pu_prev = pu_cur;
+            pm_cur = (2.0 / 3.0 - a2) * Enext;
+        } else {
+            a += t.M;
+            a2 = a * a;
+            cl = fma(0.5, a2 - a, 1.0 / 6.0) * Ek;
+            cm = (2.0 / 3.0 - a2) * Ek;
+            cr = fma(0.5, a2 + a, 1.0 / 6.0) * Ek;
+        }
+        if (!inner) {  // wave-uniform: only windows that reach an edge
+            if (k >= t.w) cl = cm = cr = 0.0;
+            if (k < 3 || (k >= t.w - 3 && k < t.w)) {
+                const int e = 3 * edge_slot_of(k, t.w);
+                cl = tab[e];
+                cm = tab[e + 1];
+                cr = tab[e + 2];
+            }
+        }
         um[s] = cm;
         if (s == 0) {
             cl0 = cl;
