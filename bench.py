@@ -193,9 +193,12 @@ def surrogate_leg(pkg, y, args):
     S = pkg.surrogate
     s, fit = S.fit(y, n_train=2_000_000, epochs=args.surrogate_epochs, seed=11,
                    log=lambda m: print(m, file=sys.stderr, flush=True))
-    # the gate's spread threshold, calibrated on a held-out sample (its own
-    # seed) so that surrogate-priced options stay within 0.7e-3 of the tree
-    cal = S.calibrate_threshold(s, y, n=250_000, seed=77, target=1e-3, margin=0.7)
+    # the gate's spread threshold, calibrated on a held-out 1M sample (its own
+    # seed) so that surrogate-priced options stay within half the 1e-3 budget
+    # of the tree.  Calibrated on 250,000 options at 0.7e-3, one network let
+    # an option through at 1.46e-3 on the evaluation mix; at 1M and 0.5e-3
+    # the same network's worst accepted option was 0.49e-3.
+    cal = S.calibrate_threshold(s, y, n=1_000_000, seed=77, target=1e-3, margin=0.5)
     gated = S.with_threshold(s, cal["threshold"], 1 << 20, y.N)
     ev = S.evaluate(gated, y, n=1_000_000, seed=5, reps=max(1, min(args.steps, 10)))
     gated.close()

@@ -226,8 +226,8 @@ def fit(y, n_train: int = 2_000_000, epochs: int = 40, seed: int = 11, infer_cap
     return s, rec
 
 
-def calibrate_threshold(s: Surrogate, y, n: int = 250_000, seed: int = 77, target: float = 1e-3,
-                        margin: float = 0.7) -> dict:
+def calibrate_threshold(s: Surrogate, y, n: int = 1_000_000, seed: int = 77, target: float = 1e-3,
+                        margin: float = 0.5) -> dict:
     """The largest ensemble-spread threshold at which every option the
     surrogate would price in a held-out sample (box_mix, its own seed) stays
     within margin * target of the FP64 tree.  In-box options are sorted by

@@ -4,10 +4,10 @@ Training workload: 2,000,000 box samples priced by the FP64 tree, a 4-member
 ensemble (8 -> 256 x 4 -> 1, SiLU) trained with the project's BF16 MFMA
 forward/backward and FP32 Adam at batch 8,192, for 120 epochs (the bench's
 count).  At 40 epochs the loss ends near 4.5e-5 and about 400 of 237,500
-in-box calibration options miss the half budget with a small spread; the
-calibrated threshold then swings between 38% and 51% acceptance with
-1-ulp changes in the tree's labels.  At 120 epochs (loss near 5e-6) 14 miss
-and 85-91% are accepted.
+in-box options of a 250,000-option calibration sample miss the half budget
+with a small spread; the calibrated threshold then swings between 38% and
+51% acceptance with 1-ulp changes in the tree's labels.  At 120 epochs (loss
+near 5e-6) 14 miss and 85-94% are accepted.
 
 Inference workload: 1,000,000 options, 95% inside the training box and 5%
 outside it (a in [0.2, 0.4] or expiry 30-40 y); ensemble inference with the
@@ -18,8 +18,8 @@ Bars:
 - training: the last epoch's loss below 5% of the first;
 - gradients on one full 8,192-row batch: per tensor within 1e-2 (relative,
   L2) of the BF16-emulating CPU reference (tests/surrogate_ref.py);
-- the gate's spread threshold is calibrated on a held-out sample of 250,000
-  options (its own seed) at half the error budget, 0.5e-3;
+- the gate's spread threshold is calibrated on a held-out sample of 1,000,000
+  options (its own seed) at half the error budget, 0.5e-3, as the bench does;
 - every one of the 1,000,000 prices within 1e-3 per 100 notional of the FP64
   tree, and a stratified subset (network-priced, re-priced in-box and
   out-of-box options) within 1e-3 of the oracle (FP64 seq restatement);
@@ -90,7 +90,7 @@ def test_gradients_on_a_full_batch_match_cpu_reference(pkg, trained):
 def test_one_million_mix_within_bar_of_tree_and_oracle(pkg, oracle, trained):
     y, s, _ = trained
     S, capi = pkg.surrogate, pkg.capi
-    cal = S.calibrate_threshold(s, y, n=250_000, seed=77, target=BAR, margin=0.5)
+    cal = S.calibrate_threshold(s, y, n=N_MIX, seed=77, target=BAR, margin=0.5)
     gated = S.with_threshold(s, cal["threshold"], N_MIX, y.N)
     mix = pkg.datasets.box_mix(5, N_MIX)
     dev = capi.to_device(mix, y)
