@@ -92,19 +92,17 @@ enum Tally {
     T_XUNIT = 66,
     T_XPLAIN = 68,
     T_HOST = 72,  // [0, T_HOST) is copied to the host by hwt_plan_sync
-    // the bucket histogram (k_setup's atomics) and the scatter cursors
-    // (k_order's), cleared by the first pricing kernel of each execution
+    // the bucket histogram (k_setup's atomics), cleared by the first pricing
+    // kernel of each execution
     T_HIST = 72,
-    T_CURSOR = T_HIST + NBUCKET,
-    T_ALLOC = T_CURSOR + NBUCKET
+    T_ALLOC = T_HIST + NBUCKET
 };
 
-// The bucket histogram and scatter cursors (in the tally) back to zero for
-// the next execution: their last readers, k_order's blocks, are done once a
-// pricing kernel runs.
+// The bucket histogram (in the tally) back to zero for the next execution:
+// its last readers, k_order's blocks, are done once a pricing kernel runs.
 __device__ __forceinline__ void clear_bucket_counts(int32_t *tally)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * NBUCKET; i += gridDim.x * blockDim.x)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NBUCKET; i += gridDim.x * blockDim.x)
         tally[T_HIST + i] = 0;
 }
 

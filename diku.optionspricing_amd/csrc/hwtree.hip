@@ -3,12 +3,13 @@
 // Pipeline of one execution (captured once into a HIP graph per plan):
 //   k_setup    one thread per option: tree constants (the reference's
 //              OptionConstants), P(e dt) and P(n dt) from the yield curve, edge
-//              factors, work estimate; trivial options are priced right here
+//              factors, work estimate, bucket and rank inside the bucket;
+//              trivial options are priced right here
 //   k_order    bucket offsets, largest work first (longest-processing-time
 //              order; each block scans the histogram itself), and the option
 //              indices into the task list; it clears the queue heads and the
 //              setup accumulators for the next execution, and the first
-//              pricing kernel clears the histogram and cursors, so no clearing
+//              pricing kernel clears the histogram, so no clearing
 //              kernel or memset node runs per execution (k_reset only runs for
 //              an empty batch, which has no k_setup)
 //   k_price_wide / k_price_narrow  persistent kernels, launched side by side:
@@ -116,8 +117,8 @@ __device__ __forceinline__ int scan_bucket(int p, int asc)
 // exclusive offsets of the buckets in scan order, into soffs (LDS).  The
 // writer block also puts the pads into the task list, the class boundaries
 // and counts into the host-visible tally, and clears the queue heads and the
-// setup accumulators for the next execution; the histogram and the scatter
-// cursors, which every block reads, are cleared by the first pricing kernel.  Each part of the
+// setup accumulators for the next execution; the histogram, which every
+// block reads, is cleared by the first pricing kernel.  Each part of the
 // row class (one K) is padded to a multiple of four tasks and each part of the
 // flat class (one width w) to a multiple of 64 / w, with idle slots (order
 // -1), so every group a wave takes shares one K or one width and an option's
@@ -129,9 +130,10 @@ __device__ __forceinline__ void scan_offsets(const int32_t *__restrict__ hist, i
 {
     const int tid = threadIdx.x;
     constexpr int NW = NBUCKET - B_WIDE, NM = B_WIDE - B_MID, NR = B_MID - B_ROW;
-    __shared__ int32_t ptot[NPARTS];
-    __shared__ int32_t part[SETUP_BLOCK];
-    __shared__ int32_t npad;
+    __shared__ int32_t ptot[NPARTS];  // options per part
+    __shared__ int32_t ppad[NPARTS];  // idle slots that round each part up to whole groups
+    __shared__ int32_t wtot[SETUP_BLOCK / WAVE];
+    __shared__ int32_t nrow_end;
     int idx[SCAN_PER], cnt[SCAN_PER];
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
@@ -139,34 +141,40 @@ __device__ __forceinline__ void scan_offsets(const int32_t *__restrict__ hist, i
         cnt[q] = hist[idx[q]];
     }
     if (tid < NPARTS) ptot[tid] = 0;
-    if (tid == 0) npad = 0;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q)
         if (idx[q] < B_MID && cnt[q]) atomicAdd(&ptot[part_of(idx[q])], cnt[q]);
     __syncthreads();
+    // one thread per part works out its pad (a division by the part's group)
+    if (tid < NPARTS) {
+        const int g = part_group(tid);
+        ppad[tid] = classes ? (g - ptot[tid] % g) % g : 0;
+    }
+    __syncthreads();
     // the lowest bucket of each part comes last in the scan and takes the pad
-    int v[SCAN_PER], pad[SCAN_PER], sum = 0, pads = 0;
+    int v[SCAN_PER], pad[SCAN_PER], sum = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
         const int b = idx[q];
         const bool lowest = b < B_MID && (b >= B_ROW ? (b - B_ROW) % ROW_HBUCKETS == 0 : b % FLAT_HBUCKETS == 0);
-        const int g = b < B_MID ? part_group(part_of(b)) : 1;
-        pad[q] = (classes && lowest) ? (g - ptot[part_of(b)] % g) % g : 0;
+        pad[q] = lowest ? ppad[part_of(b)] : 0;
         v[q] = cnt[q] + pad[q];
         sum += v[q];
-        pads += pad[q];
     }
-    part[tid] = sum;
-    if (pads) atomicAdd(&npad, pads);
+    // exclusive scan of the per-thread sums: inside each wave, then across
+    // the block's waves
+    const int lane = tid % WAVE, wv = tid / WAVE;
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int t = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += t;
+    }
+    if (lane == WAVE - 1) wtot[wv] = incl;
     __syncthreads();
-    for (int off = 1; off < SETUP_BLOCK; off <<= 1) {
-        const int32_t add = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += add;
-        __syncthreads();
-    }
-    int run = part[tid] - sum;
+    int run = incl - sum;
+    for (int k = 0; k < wv; ++k) run += wtot[k];
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
         const int p = tid * SCAN_PER + q;
@@ -176,22 +184,27 @@ __device__ __forceinline__ void scan_offsets(const int32_t *__restrict__ hist, i
             for (int j = 0; j < pad[q]; ++j) order[run - pad[q] + j] = -1;
             if (p == NW - 1) tally[T_NLARGE] = run;
             if (p == NW + NM - 1) tally[T_NMID] = run;
-            if (p == NW + NM + NR - 1) tally[T_NROW] = run;
+            if (p == NW + NM + NR - 1) {
+                tally[T_NROW] = run;
+                nrow_end = run;
+            }
             if (p == NBUCKET - 1) tally[T_NTASK] = run;
         }
     }
-    __syncthreads();  // orders the boundary writes above before thread 0 reads them
+    __syncthreads();  // orders nrow_end above before thread 0 reads it
     if (writer && tid == 0) {
         // flat parts in task order, widest first: first task and first group
-        int start = tally[T_NROW], group = 0;
+        int start = nrow_end, group = 0, npad = 0;
+#pragma unroll
+        for (int k = 0; k < NPARTS; ++k) npad += ppad[k];
+#pragma unroll
         for (int j = 0; j < FLAT_PARTS; ++j) {
             const int fp = FLAT_PARTS - 1 - j;
-            const int g = flat_per_wave(flat_width(fp));
-            const int size = ptot[ROW_MAXK + fp] + (classes ? (g - ptot[ROW_MAXK + fp] % g) % g : 0);
+            const int size = ptot[ROW_MAXK + fp] + ppad[ROW_MAXK + fp];
             tally[T_FLAT_START + j] = start;
             tally[T_FLAT_GROUP + j] = group;
             start += size;
-            group += size / g;
+            group += size / flat_per_wave(flat_width(fp));
         }
         tally[T_FLAT_START + FLAT_PARTS] = start;
         tally[T_FLAT_GROUP + FLAT_PARTS] = group;
@@ -266,10 +279,10 @@ __device__ __forceinline__ void copy_out(u32x4 *__restrict__ dst, const u32x4 *s
 __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, int32_t sort, int32_t maxWidth,
                                                        double unit_dmin, int32_t classes,
                                                        OptConst *__restrict__ cst, double *__restrict__ edge,
-                                                       int32_t *__restrict__ bucket, int32_t *__restrict__ hist,
+                                                       int2 *__restrict__ slot, int32_t *__restrict__ hist,
                                                        int32_t *__restrict__ tally, double *__restrict__ out)
 {
-    __shared__ int32_t lh[NBUCKET];
+    __shared__ int32_t lh[NBUCKET];  // the block's histogram, then its first rank in each bucket
     __shared__ unsigned long long lsum[3];  // work, seq work, wide work
     __shared__ int32_t lbad[2];             // invalid, unsupported
     __shared__ double yr[SETUP_YMAX];       // the yield curve, staged once per block
@@ -323,6 +336,7 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     for (int k = 0; k < EDGE_TAB; ++k) tab[k] = 0.0;
     unsigned long long work = 0, seq_work = 0, wide_work = 0;
     bool invalid = false, unsupported = false;
+    int b = -1, r = 0;  // bucket, rank among the block's options in it
     if (mine) {
         int32_t st;
         if (o.consts) {
@@ -334,7 +348,6 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
             st = option_constants(strike, mat, len, tu, ts, a, sigma, type, maxWidth, c);
         }
         c.status = st;
-        int b = -1;
         if (st == ST_INVALID) {
             out[i] = NAN;
             invalid = true;
@@ -385,11 +398,10 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
                 else if ((classes & CLASS_FLAT) && c.w <= FLAT_MAX_WIDTH) b = flat_bucket(c.w, c.n);
                 else if (c.w > ROW_MAX_WIDTH || !(classes & CLASS_ROWS)) b = B_MID + log_bucket(key, 4, B_WIDE - B_MID);
                 else b = row_bucket(c.w, c.n);
-                atomicAdd(&lh[b], 1);
+                r = atomicAdd(&lh[b], 1);
                 if (wide) wide_work = work;
             }
         }
-        bucket[i] = b;
     }
     // the block's tallies: one LDS atomic per wave and counter (every lane
     // adding to one LDS address took 4.5 us of the kernel)
@@ -404,6 +416,16 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
         if (wide_work) atomicAdd(&lsum[2], wide_work);
         if (n_invalid) atomicAdd(&lbad[0], n_invalid);
         if (n_unsup) atomicAdd(&lbad[1], n_unsup);
+    }
+    // The block's slots in each bucket: the global histogram's old value is
+    // the block's first rank there.  Requested now, read after the copy-out
+    // below, so the atomics' round trip overlaps it.
+    __syncthreads();
+    int first_rank[SCAN_PER];
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        const int k = tid + q * SETUP_BLOCK;
+        first_rank[q] = lh[k] ? atomicAdd(&hist[k], lh[k]) : 0;
     }
     // the wave's records out through its LDS slice: edge tables in two rounds
     // of 32 lanes, then the constants
@@ -434,9 +456,10 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
         copy_out<WAVE * REC_CST>(reinterpret_cast<u32x4 *>(cst) + (size_t)first * REC_CST, slice, count * REC_CST,
                                  lane);
     }
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) lh[tid + q * SETUP_BLOCK] = first_rank[q];
     __syncthreads();
-    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
-        if (lh[k]) atomicAdd(&hist[k], lh[k]);
+    if (mine) slot[i] = make_int2(b, b >= 0 ? lh[b] + r : 0);
     if (tid == 0) {
         if (lsum[0]) atomicAdd((unsigned long long *)&tally[A_WORK], lsum[0]);
         if (lsum[1]) atomicAdd((unsigned long long *)&tally[A_SEQWORK], lsum[1]);
@@ -447,32 +470,24 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
 }
 
 // After k_setup: the bucket offsets (every block scans the 8 KB histogram
-// itself, scan_offsets; block 0 also writes the pads and the tally), then the
-// option indices into the task list, bucket by bucket.  Ranks inside a block
-// come from LDS atomics; one global atomic per (block, bucket) reserves the
-// block's slots.  Order inside a bucket is arbitrary (options are
-// independent).  One kernel instead of a scan kernel and a scatter kernel
-// saves a kernel boundary per execution.
+// itself, scan_offsets; block 0 also writes the pads and the tally), then
+// each option's index into the task list at its bucket's offset plus the
+// rank k_setup gave it there.  Order inside a bucket is the order in which
+// k_setup's blocks reached the histogram (options are independent).  One
+// kernel instead of a scan kernel and a scatter kernel saves a kernel
+// boundary per execution.
 __global__ __launch_bounds__(SETUP_BLOCK) void k_order(int32_t n, const int32_t *__restrict__ count,
-                                                       const int32_t *__restrict__ bucket,
-                                                       const int32_t *__restrict__ hist, int32_t *__restrict__ cursor,
-                                                       int32_t *__restrict__ order, int32_t *__restrict__ tally,
-                                                       int32_t asc, int32_t classes)
+                                                       const int2 *__restrict__ slot,
+                                                       const int32_t *__restrict__ hist, int32_t *__restrict__ order,
+                                                       int32_t *__restrict__ tally, int32_t asc, int32_t classes)
 {
-    __shared__ int32_t soffs[NBUCKET], lcount[NBUCKET], lbase[NBUCKET];
+    __shared__ int32_t soffs[NBUCKET];
     const int tid = threadIdx.x;
     const int i = tid + blockIdx.x * SETUP_BLOCK;
     if (count) n = min(*count, n);
-    const int b = (i < n) ? bucket[i] : -1;  // requested first, so it arrives during the scan
-    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK) lcount[k] = 0;
+    const int2 s = (i < n) ? slot[i] : make_int2(-1, 0);  // requested first, so it arrives during the scan
     scan_offsets(hist, soffs, order, tally, asc, classes, blockIdx.x == 0);
-    __syncthreads();
-    const int r = (b >= 0) ? atomicAdd(&lcount[b], 1) : 0;
-    __syncthreads();
-    for (int k = tid; k < NBUCKET; k += SETUP_BLOCK)
-        if (lcount[k]) lbase[k] = atomicAdd(&cursor[k], lcount[k]);
-    __syncthreads();
-    if (b >= 0) order[soffs[b] + lbase[b] + r] = i;
+    if (s.x >= 0) order[soffs[s.x] + s.y] = i;
 }
 
 // What the wide-tree kernel's stencil steps executed in the last execution,
@@ -645,7 +660,8 @@ struct hwt_plan {
     size_t ws_bytes = 0;
     OptConst *cst = nullptr;
     double *edge = nullptr;
-    int32_t *bucket = nullptr, *order = nullptr, *hist = nullptr, *cursor = nullptr, *tally = nullptr;
+    int2 *slot = nullptr;  // per option: bucket, rank inside it
+    int32_t *order = nullptr, *hist = nullptr, *tally = nullptr;
     int32_t *h_tally = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     hipEvent_t evw0 = nullptr, evw1 = nullptr, evn0 = nullptr, evn1 = nullptr;
@@ -690,9 +706,8 @@ static int launch_pipeline(hwt_plan *p, const DevOpts &o, const DevYield &y, dou
         // form 1 prices trees up to its column length; wider ones are unsupported
         const int max_width = p->cfg.form == HWT_FORM_LANE ? p->lane_width : WAVE_MAX_WIDTH;
         k_setup<<<blocks, SETUP_BLOCK, 0, s>>>(o, y, sort, max_width, unit_dmin, classes, p->cst, p->edge,
-                                               p->bucket, p->hist, p->tally, out);
-        k_order<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->bucket, p->hist, p->cursor, p->order, p->tally, asc,
-                                               classes);
+                                               p->slot, p->hist, p->tally, out);
+        k_order<<<blocks, SETUP_BLOCK, 0, s>>>(n, o.count, p->slot, p->hist, p->order, p->tally, asc, classes);
         if (time_kernel) HWT_HIP(hipEventRecord(p->evk0, s));
         if (p->cfg.form == HWT_FORM_LANE) {
             const int lb = p->lane_block;
@@ -871,9 +886,9 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     };
     const size_t o_cst = carve(cap * sizeof(OptConst));
     const size_t o_edge = carve(cap * EDGE_TAB * sizeof(double));
-    const size_t o_bucket = carve(cap * sizeof(int32_t));
+    const size_t o_slot = carve(cap * sizeof(int2));
     const size_t o_order = carve((cap + MAX_PADS) * sizeof(int32_t));  // + row and flat pads
-    const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));  // with the histogram and scatter cursors
+    const size_t o_tally = carve(T_ALLOC * sizeof(int32_t));  // with the histogram
     const bool lane = p->cfg.form == HWT_FORM_LANE;
     // the wide-tree kernels' parked forward states (form 2 only)
     const size_t o_scw = carve(lane ? 0 : (size_t)p->grid_wide * WAVE * MAXK * sizeof(double));
@@ -891,11 +906,10 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     char *base = (char *)p->ws;
     p->cst = (OptConst *)(base + o_cst);
     p->edge = (double *)(base + o_edge);
-    p->bucket = (int32_t *)(base + o_bucket);
+    p->slot = (int2 *)(base + o_slot);
     p->order = (int32_t *)(base + o_order);
     p->tally = (int32_t *)(base + o_tally);
     p->hist = p->tally + T_HIST;
-    p->cursor = p->tally + T_CURSOR;
     p->scratch_wide = (double *)(base + o_scw);
     p->redo = p->cfg.precision == HWT_PREC_MIXED ? (int32_t *)(base + o_redo) : nullptr;
     p->cols = lane ? (double *)(base + o_cols) : nullptr;
@@ -1478,16 +1492,16 @@ int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, c
 // hwt_price_device_noalloc keeps every buffer its kernels need beyond the
 // caller's inputs and output in this one static device array, which each
 // device holds from the module's load until the process ends.  Layout (bytes,
-// each part 256-aligned): OptConst[N], edge tables[N][36], bucket[N],
-// order[N + row pads], counters with the histogram and scatter cursors, and the wide-tree kernel's parked forward states (one
+// each part 256-aligned): OptConst[N], edge tables[N][36], (bucket, rank)[N],
+// order[N + row pads], counters with the histogram, and the wide-tree kernel's parked forward states (one
 // 64 x MAXK double slot per resident wave, at most NOALLOC_WIDE_SLOTS).
 constexpr int NOALLOC_MAX_OPTIONS = HWT_NOALLOC_MAX_OPTIONS;
 constexpr int NOALLOC_WIDE_SLOTS = 1024;
 constexpr size_t noalloc_part(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 constexpr size_t NOALLOC_O_CST = 0;
 constexpr size_t NOALLOC_O_EDGE = NOALLOC_O_CST + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * sizeof(OptConst));
-constexpr size_t NOALLOC_O_BUCKET = NOALLOC_O_EDGE + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * EDGE_TAB * 8);
-constexpr size_t NOALLOC_O_ORDER = NOALLOC_O_BUCKET + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * 4);
+constexpr size_t NOALLOC_O_SLOT = NOALLOC_O_EDGE + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * EDGE_TAB * 8);
+constexpr size_t NOALLOC_O_ORDER = NOALLOC_O_SLOT + noalloc_part((size_t)NOALLOC_MAX_OPTIONS * sizeof(int2));
 constexpr size_t NOALLOC_O_TALLY = NOALLOC_O_ORDER + noalloc_part(((size_t)NOALLOC_MAX_OPTIONS + MAX_PADS) * 4);
 constexpr size_t NOALLOC_O_WIDE = NOALLOC_O_TALLY + noalloc_part(T_ALLOC * 4);
 constexpr size_t NOALLOC_BYTES = NOALLOC_O_WIDE + (size_t)NOALLOC_WIDE_SLOTS * WAVE * MAXK * 8;
@@ -1563,11 +1577,10 @@ int hwt_price_device_noalloc(const hwt_options_f64 *dopts, const hwt_yield_f64 *
     v.grid_narrow = d.grid_narrow;
     v.cst = (OptConst *)(d.ptr + NOALLOC_O_CST);
     v.edge = (double *)(d.ptr + NOALLOC_O_EDGE);
-    v.bucket = (int32_t *)(d.ptr + NOALLOC_O_BUCKET);
+    v.slot = (int2 *)(d.ptr + NOALLOC_O_SLOT);
     v.order = (int32_t *)(d.ptr + NOALLOC_O_ORDER);
     v.tally = (int32_t *)(d.ptr + NOALLOC_O_TALLY);
     v.hist = v.tally + T_HIST;
-    v.cursor = v.tally + T_CURSOR;
     v.scratch_wide = (double *)(d.ptr + NOALLOC_O_WIDE);
     hipStream_t s = (hipStream_t)stream;
     // the array keeps the last call's contents: counters start from zero by a
