@@ -19,7 +19,7 @@ LIB_PATH = PKG_DIR / "libhwtree.so"
 HWT_OK, HWT_EINVAL, HWT_EUNSUPPORTED, HWT_ENOMEM, HWT_EHIP = 0, -1, -2, -3, -4
 FORM_AUTO, FORM_LANE, FORM_WAVE = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
-FLAG_NO_GRAPH, FLAG_NO_ROWS, FLAG_CONCURRENT, FLAG_FUSED, FLAG_NO_FLAT = 1, 2, 4, 8, 16
+FLAG_NO_GRAPH, FLAG_NO_ROWS, FLAG_CONCURRENT, FLAG_FUSED, FLAG_NO_FLAT, FLAG_DIRECT = 1, 2, 4, 8, 16, 32
 
 # The exported C symbols, in include/hwtree.h order.
 EXPORTS = (

@@ -18,6 +18,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
 {
     const int nredo = uni(tally[T_NREDO]);
     price_tasks<double, MAXK, false, true>(cst, edge, redo, tally, T_QUEUE_X, 0, nredo, out, scratch, nullptr);
+    stamp_wave_end(tally);
 }
 
 } // namespace hwt

@@ -85,18 +85,26 @@ enum Tally {
     // first task and first group of each flat part (one width each, widest
     // first, as the scan lays them out), then the end of the last part
     T_FLAT_START = 32,  // FLAT_PARTS + 1 entries
+    // int64, device wall clock (constant rate) of the last execution at
+    // k_setup's start (its first block)
+    T_T0 = 44,
     T_FLAT_GROUP = 48,  // FLAT_PARTS + 1 entries
     // int64: what the windowed kernels' stencil steps executed (StepCount):
     // operations, unit-form node slots, plain-form node slots
     T_XFLOP = 64,
     T_XUNIT = 66,
     T_XPLAIN = 68,
-    T_HOST = 72,  // [0, T_HOST) is copied to the host by hwt_plan_sync
+    // int64 x N_END: the latest end of a pricing wavefront, wavefront w in
+    // slot w % N_END (one address for all of them serialised 4,096 atomics
+    // into 37 us); cleared by k_order's writer block, the maximum taken at sync
+    T_END = 72,
+    T_HOST = T_END + 2 * 256,  // [0, T_HOST) is copied to the host by hwt_plan_sync
     // the bucket histogram (k_setup's atomics), cleared by the first pricing
     // kernel of each execution
-    T_HIST = 72,
+    T_HIST = T_HOST,
     T_ALLOC = T_HIST + NBUCKET
 };
+constexpr int N_END = (T_HOST - T_END) / 2;
 
 // The bucket histogram (in the tally) back to zero for the next execution:
 // its last readers, k_order's blocks, are done once a pricing kernel runs.
@@ -104,6 +112,17 @@ __device__ __forceinline__ void clear_bucket_counts(int32_t *tally)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NBUCKET; i += gridDim.x * blockDim.x)
         tally[T_HIST + i] = 0;
+}
+
+// The execution's latest pricing-wavefront end so far, for
+// hwt_stats.total_ms: one 64-bit atomic per wavefront, as it exits, into the
+// wavefront's end slot.
+__device__ __forceinline__ void stamp_wave_end(int32_t *tally)
+{
+    const int w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (threadIdx.x % WAVE == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(tally + T_END) + w % N_END,
+                  (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
 // Pricing kernels (hwt_pricers.h).  All are persistent: one single-wave

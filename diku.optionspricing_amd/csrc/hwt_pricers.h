@@ -198,6 +198,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     clear_bucket_counts(tally);
     const int nlarge = uni(tally[T_NLARGE]);
     price_tasks<T, MAXK, false, true>(cst, edge, order, tally, T_QUEUE, 0, nlarge, out, scratch, redo);
+    stamp_wave_end(tally);
 }
 
 
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
     price_tasks<T, KSMALL, true, false>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr, redo);
     price_row_groups<T, false>(cst, edge, order, nmid, nrow, out, tally, redo);
     price_flat_groups<T, false>(cst, edge, order, out, tally, redo);
+    stamp_wave_end(tally);
 }
 
 // Every class in one persistent kernel: the wide trees from the LPT queue,
@@ -357,6 +359,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2))) void 
     price_tasks<T, KSMALL, true, true>(cst, edge, order, tally, T_QUEUE_S, nlarge, nmid, out, nullptr, redo);
     price_row_groups<T, true>(cst, edge, order, nmid, nrow, out, tally, redo);
     price_flat_groups<T, true>(cst, edge, order, out, tally, redo);
+    stamp_wave_end(tally);
 }
 
 } // namespace hwt

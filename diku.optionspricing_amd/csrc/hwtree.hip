@@ -101,6 +101,9 @@ constexpr int SETUP_BLOCK = 256;
 constexpr int SETUP_YMAX = 2048;  // knots staged in LDS by k_setup (24 KB)
 constexpr int SCAN_PER = NBUCKET / SETUP_BLOCK;
 static_assert(NBUCKET % SETUP_BLOCK == 0, "scan positions per thread");
+static_assert(T_FLAT_START + FLAT_PARTS + 1 <= T_T0 && T_T0 + 2 <= T_FLAT_GROUP &&
+                  T_FLAT_GROUP + FLAT_PARTS + 1 <= T_XFLOP && T_XPLAIN + 2 <= T_END && N_END <= SETUP_BLOCK,
+              "tally layout");
 
 // Bucket at scan position p: the wide class, then the mid class, each largest
 // key first unless an ascending order was asked for, then the row class by
@@ -192,6 +195,7 @@ __device__ __forceinline__ void scan_offsets(const int32_t *__restrict__ hist, i
         }
     }
     __syncthreads();  // orders nrow_end above before thread 0 reads it
+    if (writer && tid < N_END) reinterpret_cast<unsigned long long *>(tally + T_END)[tid] = 0;
     if (writer && tid == 0) {
         // flat parts in task order, widest first: first task and first group
         int start = nrow_end, group = 0, npad = 0;
@@ -293,6 +297,8 @@ __global__ __launch_bounds__(SETUP_BLOCK) void k_setup(DevOpts o, DevYield y, in
     const int i = tid + blockIdx.x * SETUP_BLOCK;
     const int nopt = o.count ? min(*o.count, o.n) : o.n;
     const bool mine = i < nopt;
+    if (i == 0)
+        reinterpret_cast<unsigned long long *>(tally)[T_T0 / 2] = __builtin_amdgcn_s_memrealtime();
     // the option's fields are requested before the curve is staged, so the
     // two loads overlap
     double strike = 0.0, mat = 0.0, len = 0.0, a = 0.0, sigma = 0.0;
@@ -534,13 +540,14 @@ __global__ void k_price_lane(const OptConst *__restrict__ cst, const double *__r
 {
     clear_bucket_counts(tally);
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= tally[T_NTASK]) return;
-    const int o = order[gid];
-    if (o < 0) return;  // idle pad slot
-    const OptConst c = cst[o];
-    double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)colw * WAVE) + (gid % WAVE);
-    out[o] = lane_price(c, edge + (size_t)o * EDGE_TAB, LaneColumn{base},
-                        LaneColumn{base + (size_t)colw * WAVE});
+    const int o = gid < tally[T_NTASK] ? order[gid] : -1;  // -1: past the end or an idle pad slot
+    if (o >= 0) {
+        const OptConst c = cst[o];
+        double *base = cols + (size_t)(gid / WAVE) * (2 * (size_t)colw * WAVE) + (gid % WAVE);
+        out[o] = lane_price(c, edge + (size_t)o * EDGE_TAB, LaneColumn{base},
+                            LaneColumn{base + (size_t)colw * WAVE});
+    }
+    stamp_wave_end(tally);
 }
 
 // ------------------------------------------------------------- alphas
@@ -663,7 +670,8 @@ struct hwt_plan {
     int2 *slot = nullptr;  // per option: bucket, rank inside it
     int32_t *order = nullptr, *hist = nullptr, *tally = nullptr;
     int32_t *h_tally = nullptr;  // pinned
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+    hipEvent_t evk0 = nullptr, evk1 = nullptr;
+    double wall_khz = 0.0;  // rate of the device wall clock the kernels stamp
     hipEvent_t evw0 = nullptr, evw1 = nullptr, evn0 = nullptr, evn1 = nullptr;
     // captured graph and the arguments it was captured with
     hipGraphExec_t exec = nullptr;
@@ -920,8 +928,11 @@ int hwt_plan_create(hwt_plan **out, int32_t capacity, int32_t yield_capacity, co
     if (hipHostMalloc((void **)&p->h_tally, T_HOST * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(HWT_ENOMEM, "hipHostMalloc failed");
     std::memset(p->h_tally, 0, T_HOST * sizeof(int32_t));  // a plan never executed syncs clean
-    if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
-        hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device) != hipSuccess || khz <= 0)
+        return bail(HWT_EHIP, "the device wall clock rate is unavailable");
+    p->wall_khz = khz;
+    if (hipEventCreate(&p->evk0) != hipSuccess || hipEventCreate(&p->evk1) != hipSuccess ||
         hipEventCreate(&p->evw0) != hipSuccess || hipEventCreate(&p->evw1) != hipSuccess ||
         hipEventCreate(&p->evn0) != hipSuccess || hipEventCreate(&p->evn1) != hipSuccess)
         return bail(HWT_EHIP, "hipEventCreate failed");
@@ -936,8 +947,6 @@ void hwt_plan_destroy(hwt_plan *p)
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    if (p->ev0) (void)hipEventDestroy(p->ev0);
-    if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->evk0) (void)hipEventDestroy(p->evk0);
     if (p->evk1) (void)hipEventDestroy(p->evk1);
     for (hipEvent_t e : {p->evw0, p->evw1, p->evn0, p->evn1})
@@ -961,9 +970,11 @@ static int plan_execute_views(hwt_plan *p, const DevOpts &o, const DevYield &y, 
     p->last_stream = s;
     p->last_n = o.n;
     p->steps_counted = false;
-    HWT_HIP(hipEventRecord(p->ev0, s));
-    if (p->cfg.flags & HWT_FLAG_NO_GRAPH) {
-        const int rc = launch_pipeline(p, o, y, dout, s, true);
+    // no event records around the execution: the kernels stamp its start and
+    // end (T_T0, T_T1), which costs nothing between back-to-back executions;
+    // two event records per execution measured 9 us per graph replay
+    if (p->cfg.flags & (HWT_FLAG_NO_GRAPH | HWT_FLAG_DIRECT)) {
+        const int rc = launch_pipeline(p, o, y, dout, s, (p->cfg.flags & HWT_FLAG_NO_GRAPH) != 0);
         if (rc) return rc;
     } else {
         const bool same = p->exec && p->g_stream == s && p->g_out == dout &&
@@ -992,7 +1003,6 @@ static int plan_execute_views(hwt_plan *p, const DevOpts &o, const DevYield &y, 
         HWT_HIP(hipGraphLaunch(p->exec, s));
         p->timed_kernel = p->timed_split = false;
     }
-    HWT_HIP(hipEventRecord(p->ev1, s));
     return HWT_OK;
 }
 
@@ -1062,7 +1072,11 @@ int hwt_plan_sync(hwt_plan *p, hwt_stats *stats)
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p->ev0, p->ev1) == hipSuccess) stats->total_ms = ms;
+        const unsigned long long *t64u = reinterpret_cast<const unsigned long long *>(t);
+        const unsigned long long t0 = t64u[T_T0 / 2];
+        unsigned long long t1 = 0;
+        for (int k = 0; k < N_END; ++k) t1 = std::max(t1, t64u[T_END / 2 + k]);
+        stats->total_ms = (t0 && t1 > t0) ? (double)(t1 - t0) / p->wall_khz : 0.0;
         stats->kernel_ms = stats->total_ms;
         if (p->timed_kernel) {
             if (hipEventElapsedTime(&ms, p->evk0, p->evk1) == hipSuccess) stats->kernel_ms = ms;
@@ -1642,6 +1656,8 @@ namespace {
 std::mutex g_tune_mu;
 std::map<std::string, hwt_config> g_tune_cache;
 bool g_tune_loaded = false;
+// the flags the tuner chooses (the caller's others are kept as given)
+constexpr int TUNED_FLAGS = HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT | HWT_FLAG_DIRECT;
 
 int ilog2(int64_t v)
 {
@@ -1679,19 +1695,23 @@ void tune_store_file(const std::string &key, const hwt_config &c)
     const char *path = getenv("HWT_TUNE_CACHE");
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
-    const int kept = c.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT);
+    const int kept = c.flags & TUNED_FLAGS;
     f << key << ' ' << c.form << ' ' << c.sort << ' ' << kept << ' '
       << c.wide_per_cu << ' ' << c.narrow_per_cu << '\n';
 }
 
-// Median pipeline time of seven timed executions after five warm-ups, in the
-// mode the caller's plans will run in (graph replays unless HWT_FLAG_NO_GRAPH;
-// the first warm-up captures the graph).  warm_ms extends the warm-up to at
+// Pipeline time per execution, in the mode the caller's plans will run in
+// (graph replays unless HWT_FLAG_NO_GRAPH or HWT_FLAG_DIRECT; the first
+// warm-up captures the graph): five warm-ups, each synchronised, then seven
+// bursts of four executions queued back to back, timed by HIP events around
+// each burst; the median burst's time per execution.  Back-to-back periods
+// include what a launch costs between executions, which the kernels' own
+// stamps (hwt_stats.total_ms) leave out.  warm_ms extends the warm-up to at
 // least that much wall time: the first timing of a tuning run would otherwise
-// catch the clocks still ramping and read several percent slow.  A
-// candidate whose second execution (the first one after the graph capture)
-// already takes longer than cutoff_ms is out: that time is returned at once.
-constexpr int TUNE_WARM = 5, TUNE_TIMED = 7;
+// catch the clocks still ramping and read several percent slow.  A candidate
+// whose second execution (the first one after the graph capture) already
+// takes longer than cutoff_ms is out: that time is returned at once.
+constexpr int TUNE_WARM = 5, TUNE_TIMED = 7, TUNE_BURST = 4;
 int tune_time(const hwt_options_f64 *o, const hwt_yield_f64 *y, double *out, hwt_config c, double &ms,
               hwt_stats *first, double warm_ms = 0.0, double cutoff_ms = 0.0)
 {
@@ -1717,11 +1737,25 @@ int tune_time(const hwt_options_f64 *o, const hwt_yield_f64 *y, double *out, hwt
         if (i + 1 >= TUNE_WARM && el >= warm_ms) break;
     }
     double t[TUNE_TIMED];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == HWT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(HWT_EHIP, "hipEventCreate failed");
     for (int i = 0; i < TUNE_TIMED && rc == HWT_OK; ++i) {
-        hwt_stats st{};
-        rc = run(st);
-        t[i] = st.total_ms;
+        float el = 0.f;
+        if (hipEventRecord(e0, p->stream) != hipSuccess) rc = fail(HWT_EHIP, "hipEventRecord failed");
+        for (int r = 0; r < TUNE_BURST && rc == HWT_OK; ++r) rc = hwt_plan_execute(p, o, y, out, nullptr);
+        if (rc == HWT_OK && (hipEventRecord(e1, p->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&el, e0, e1) != hipSuccess))
+            rc = fail(HWT_EHIP, "timing a burst of executions failed");
+        t[i] = el / TUNE_BURST;
     }
+    if (rc == HWT_OK) {
+        hwt_stats st{};
+        rc = hwt_plan_sync(p, &st);
+        if (rc == HWT_EUNSUPPORTED) rc = HWT_OK;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     hwt_plan_destroy(p);
     if (rc) return rc;
     std::sort(t, t + TUNE_TIMED);
@@ -1761,8 +1795,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
             hwt_config c = base;
             c.form = it->second.form;
             c.sort = it->second.sort;
-            c.flags = (base.flags & ~(HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT)) |
-                      (it->second.flags & (HWT_FLAG_NO_ROWS | HWT_FLAG_CONCURRENT | HWT_FLAG_FUSED | HWT_FLAG_NO_FLAT));
+            c.flags = (base.flags & ~TUNED_FLAGS) | (it->second.flags & TUNED_FLAGS);
             c.wide_per_cu = it->second.wide_per_cu;
             c.narrow_per_cu = it->second.narrow_per_cu;
             *cfg = c;
@@ -1776,7 +1809,8 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         if (report) *report = rep;
         return HWT_OK;
     }
-    // greedy coordinate descent; a change must win by 2% to be kept
+    // greedy coordinate descent; a change must win by 1% to be kept (burst
+    // medians repeat within about 0.2%)
     hwt_config best = base;
     double best_ms = base_ms;
     int cand = 1;
@@ -1785,7 +1819,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         const int r = tune_time(dev_opts, dev_yield, dev_out, c, ms, nullptr, 0.0, 1.5 * best_ms);
         if (r) return r;
         ++cand;
-        if (ms < 0.98 * best_ms) {
+        if (ms < 0.99 * best_ms) {
             best = c;
             best_ms = ms;
         }
@@ -1797,7 +1831,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         c.sort = so;
         if ((rc = consider(c))) return rc;
     }
-    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_NO_FLAT, HWT_FLAG_CONCURRENT, HWT_FLAG_FUSED}) {
+    for (int bit : {HWT_FLAG_NO_ROWS, HWT_FLAG_NO_FLAT, HWT_FLAG_CONCURRENT, HWT_FLAG_FUSED, HWT_FLAG_DIRECT}) {
         hwt_config c = best;
         c.flags ^= bit;
         if ((rc = consider(c))) return rc;
@@ -1831,7 +1865,7 @@ extern "C" int hwt_autotune(const hwt_options_f64 *dev_opts, const hwt_yield_f64
         if ((rc = tune_time(dev_opts, dev_yield, dev_out, best, again_best, nullptr))) return rc;
         cand += 2;
         rep.default_ms = again_base;  // the fair pair is what the report compares
-        if (!(again_best < 0.98 * again_base)) {
+        if (!(again_best < 0.99 * again_base)) {
             best = base;
             best_ms = again_base;
         } else {

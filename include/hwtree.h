@@ -115,12 +115,23 @@ typedef struct {
  * wavefront with HWT_FLAG_NO_ROWS) instead of side by side across a whole
  * wavefront, one node per lane (the flattened form). */
 #define HWT_FLAG_NO_FLAT 16
+/* Launch the plan's kernels one by one, like HWT_FLAG_NO_GRAPH but without its
+ * timing events, instead of replaying the plan's HIP graph.  Back to back on a
+ * large batch the direct launches cost less per execution than a graph replay
+ * (0.273 against 0.278 ms on the 60k skewed batch); on small batches the
+ * host's launch calls cost more.  The autotuner tries both.  HWT_FLAG_NO_GRAPH
+ * takes precedence. */
+#define HWT_FLAG_DIRECT 32
 
 typedef struct {
     double  kernel_ms;        /* pricing kernels, first start to last end (HWT_FLAG_NO_GRAPH),
                                  else = total_ms */
     double  total_ms;         /* whole plan execution: constants, ordering, pricing;
-                                 excludes H2D/D2H copies.  HIP events. */
+                                 excludes H2D/D2H copies.  Device wall clock, stamped
+                                 by the kernels: the set-up kernel's first block at
+                                 its start to the last pricing wavefront's end (the
+                                 first kernel's launch is not included); 0 for an
+                                 empty batch. */
     double  wide_ms;          /* wide-tree pricing kernel alone (HWT_FLAG_NO_GRAPH), else 0 */
     double  narrow_ms;        /* narrow-tree pricing kernel alone (HWT_FLAG_NO_GRAPH), else 0 */
     int64_t device_memory;    /* bytes of device memory held by the plan */
@@ -284,13 +295,14 @@ int hwt_alphas_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield,
  * and cuda-multi/Kernel.cuh:374).  Prices the device batch repeatedly with
  * candidate settings of form 2 -- processing order of the wide trees (work,
  * height or width), narrow trees four or one per wavefront, the two pricing
- * kernels in sequence, side by side or fused, and resident workgroups per CU
- * of each -- and then form 1 instead of form 2 for FP64 batches of at most
- * 262,144 options (an inspector choosing the kernel form per batch), by
- * greedy coordinate descent from *cfg (form 0 or 2), timing each whole pipeline
- * with HIP events (median of seven after five warm-ups, replaying the plan's
- * graph unless *cfg has HWT_FLAG_NO_GRAPH), and writes the fastest back into
- * *cfg.
+ * kernels in sequence, side by side or fused, graph replays or direct
+ * launches (HWT_FLAG_DIRECT), and resident workgroups per CU of each -- and
+ * then form 1 instead of form 2 for FP64 batches of at most 262,144 options
+ * (an inspector choosing the kernel form per batch), by greedy coordinate
+ * descent from *cfg (form 0 or 2), timing each candidate with HIP events
+ * around bursts of four executions queued back to back (median of seven
+ * bursts after five warm-ups; a change must win by 1%), and writes the
+ * fastest back into *cfg.
  * The choice is cached per batch signature (precision, option count and the
  * tree-class counts and total work, each to a power of two) for the life of
  * the process, and appended to the file named by the environment variable

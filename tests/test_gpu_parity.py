@@ -284,6 +284,42 @@ def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
     plan.close()
 
 
+@pytest.mark.parametrize("flags", [0, 1, 4, 8], ids=["graph", "eager", "concurrent", "fused"])
+def test_execution_time_stamps(pkg, flags):
+    """hwt_stats.total_ms comes from the kernels' wall-clock stamps (the
+    set-up kernel's start to the last pricing wavefront's end), not from
+    events around the execution.  For back-to-back executions it must lie
+    between 40% and 105% of their period measured by events around all of
+    them, and an empty batch reports 0."""
+    import ctypes as C
+    capi, dm = pkg.capi, pkg.devmem
+    y = pkg.datasets.synthetic_yield()
+    opts = pkg.datasets.skewed(seed=6, n=6000)
+    dev = capi.to_device(opts, y)
+    plan = capi.Plan(opts.N, y.N, capi.config(flags=flags))
+    for _ in range(5):
+        plan.execute(dev)
+    plan.sync()
+    stream = C.c_void_p()
+    dm._ck(dm.hip().hipStreamCreate(C.byref(stream)))
+    reps = 20
+    e0, e1 = dm.Event(), dm.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.execute(dev, stream=stream.value)
+    e1.record(stream)
+    e1.synchronize()
+    period = e0.elapsed_ms(e1) / reps
+    st = plan.sync()
+    assert 0.4 * period <= st["total_ms"] <= 1.05 * period, (st["total_ms"], period)
+    if flags & capi.FLAG_NO_GRAPH:
+        assert 0.0 < st["kernel_ms"] <= 1.05 * period
+    plan.execute(dev, n=0)
+    assert plan.sync()["total_ms"] == 0.0
+    plan.close()
+    dm._ck(dm.hip().hipStreamDestroy(stream))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_wide_step_counts(pkg, precision):
     """The wide kernel's own count of what its stencil steps executed.  Every
