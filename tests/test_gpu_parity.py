@@ -243,7 +243,7 @@ def test_plan_graph_replays(pkg, oracle):
     plan2.close()
 
 
-@pytest.mark.parametrize("flags", [0, 1], ids=["graph", "eager"])
+@pytest.mark.parametrize("flags", [0, 1, 32], ids=["graph", "eager", "direct"])
 def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
     """One plan, executions queued back to back with no sync between them,
     then an empty batch, a batch with an invalid option and a good batch
@@ -284,7 +284,7 @@ def test_back_to_back_executions_leave_counters_clear(pkg, oracle, flags):
     plan.close()
 
 
-@pytest.mark.parametrize("flags", [0, 1, 4, 8], ids=["graph", "eager", "concurrent", "fused"])
+@pytest.mark.parametrize("flags", [0, 1, 4, 8, 32], ids=["graph", "eager", "concurrent", "fused", "direct"])
 def test_execution_time_stamps(pkg, flags):
     """hwt_stats.total_ms comes from the kernels' wall-clock stamps (the
     set-up kernel's start to the last pricing wavefront's end), not from
@@ -451,6 +451,7 @@ def test_flat_form_mixed_precision_and_packing(pkg, oracle, golden_yield):
 
 @pytest.mark.parametrize("setting", [{"flags": 2}, {"flags": 4}, {"flags": 6}, {"flags": 8}, {"flags": 10},
                                      {"flags": 9}, {"flags": 16}, {"flags": 18}, {"flags": 24},
+                                     {"flags": 32}, {"flags": 48}, {"flags": 36},
                                      {"wide_per_cu": 2, "narrow_per_cu": 4},
                                      {"sort": "H"}, {"sort": "W"}])
 def test_form2_settings_price_the_same(pkg, oracle, setting):
