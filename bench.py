@@ -5,7 +5,8 @@ Workload: the 60,000-option skewed batch (90% small trees: bond 1-5y at
 a in [0.01,0.05], widths up to 1,327 nodes, heights up to 1,080 steps),
 synthetic 120-knot yield curve, seeded inputs already resident in HBM.
 A step prices the whole batch: device-side constants and ordering plus the two
-pricing kernels, replayed from the plan's HIP graph.
+pricing kernels, replayed from the plan's HIP graph or launched directly
+(HWT_FLAG_DIRECT), whichever the autotuner measured faster.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the wide-tree
 pricing kernel, timed alone with HIP events on its stream): FP64 vector FLOPs
@@ -94,8 +95,8 @@ def cpu_baseline(pkg, opts, y, c1):
 def reference_sets_leg(pkg, capi, args):
     """The reference's seven 65,536-option benchmark distributions (drawn by
     the seeded generator; the reference's 15-knot curve), each priced whole
-    by the FP64 tree: graph replays timed with HIP events, the autotuned
-    configuration.  The reference's own figures for these sets are on a
+    by the FP64 tree: back-to-back executions timed with HIP events, in the
+    autotuned configuration (graph replays or direct launches).  The reference's own figures for these sets are on a
     GTX 780 Ti (CUDA-multi FP64, 4_SKEWED: 1.147 s)."""
     import ctypes as C
     dm = pkg.devmem
@@ -310,12 +311,13 @@ def main():
                           "wide_node_updates": st["wide_node_updates"]}), flush=True)
         return
 
-    # whole-job throughput: graph replays on a caller stream, HIP events around K steps
+    # whole-job throughput: executions on a caller stream (graph replays or direct
+    # launches, as tuned), HIP events around K steps
     import ctypes as C
     stream = C.c_void_p()
     dm._ck(dm.hip().hipStreamCreate(C.byref(stream)))
 
-    def time_graph(cfg):
+    def time_steps(cfg):
         plan = capi.Plan(opts.N, y.N, cfg)
         for _ in range(args.warmup):
             plan.execute(dev, stream=stream.value)
@@ -334,9 +336,9 @@ def main():
     # the project autotuner picks the form-2 settings for this batch first
     tuned, tune_rep = capi.autotune(dev)
     tuned_mixed, tune_rep_mixed = capi.autotune(dev, capi.config(precision=capi.PREC_MIXED))
-    ms_per_step, st, fp64_prices = time_graph(tuned)
+    ms_per_step, st, fp64_prices = time_steps(tuned)
     # mixed precision on the same batch: FP32 tree passes, FP64 discounting and sums
-    mixed_ms, _, mixed_prices = time_graph(tuned_mixed)
+    mixed_ms, _, mixed_prices = time_steps(tuned_mixed)
     mixed_err = np.abs(mixed_prices - fp64_prices)
 
     # dominant kernel on its own: the same pipeline launched without the graph,
@@ -401,7 +403,8 @@ def main():
                         "and 6,000 large (width 267-1,327, height 360-1,080)",
             "options": opts.N,
             "precision": "fp64",
-            "pipeline": "device constants + LPT ordering + wide/narrow persistent pricing kernels, one HIP graph",
+            "pipeline": "device constants + LPT ordering + wide/narrow persistent pricing kernels, "
+                        + ("launched directly" if tuned.flags & capi.FLAG_DIRECT else "one HIP graph"),
             "tuned": tuned.as_dict(),
         },
         "roofline": {
@@ -436,7 +439,7 @@ def main():
             "one_shot_host_buffers": {"options_per_s": opts.N / one_shot_s, "ms": one_shot_s * 1e3,
                                       "note": "hwt_price_f64 on host arrays: plan creation, H2D copies, "
                                               "pricing, D2H copy and plan destruction (wall clock)"},
-            "graph_total_ms_per_step": ms_per_step,
+            "total_ms_per_step": ms_per_step,
             "autotune": {"fp64": tune_rep, "mixed": tune_rep_mixed, "mixed_config": tuned_mixed.as_dict()},
             "mixed_precision": {
                 "options_per_s": opts.N / (mixed_ms * 1e-3),
