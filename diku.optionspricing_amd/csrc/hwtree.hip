@@ -1191,14 +1191,12 @@ static int host_max_width(const hwt_options_f64 *o)
 }
 
 // One execution of a fresh plan over device views, results copied to host.
-static int price_views_once(const DevOpts &o, const DevYield &y, double *dev_out, double *host_out, hwt_config c,
-                            hwt_stats *stats)
+// One execution of plan p over device views, synchronised, and the prices
+// copied back to host_out.
+static int price_views(hwt_plan *p, const DevOpts &o, const DevYield &y, double *dev_out, double *host_out,
+                       hwt_stats *stats)
 {
-    c.flags |= HWT_FLAG_NO_GRAPH;  // one execution: a graph would not be replayed
-    hwt_plan *p = nullptr;
-    int rc = hwt_plan_create(&p, o.n, y.n, &c);
-    if (rc) return rc;
-    rc = plan_execute_views(p, o, y, dev_out, p->stream);
+    const int rc = plan_execute_views(p, o, y, dev_out, p->stream);
     int rs = rc ? rc : hwt_plan_sync(p, stats);
     const std::string keep = g_last_error;
     if (rc == HWT_OK) {
@@ -1207,7 +1205,6 @@ static int price_views_once(const DevOpts &o, const DevYield &y, double *dev_out
         else
             g_last_error = keep;
     }
-    hwt_plan_destroy(p);
     return rs;
 }
 
@@ -1223,8 +1220,14 @@ struct hwt_batch {
     DevYield y{};
     double *out = nullptr;
     size_t bytes = 0;
+    // the plan of the last hwt_batch_price call, kept while the settings stay
+    // the same, so repeated pricing allocates nothing and replays its graph
+    hwt_plan *plan = nullptr;
+    hwt_config plan_cfg{};
+    bool once = false;  // a one-shot call's batch: its single execution runs without a graph
     ~hwt_batch()
     {
+        if (plan) hwt_plan_destroy(plan);
         if (cmem) (void)hipFree(cmem);
     }
 };
@@ -1379,7 +1382,10 @@ int hwt_batch_create_f32(hwt_batch **out, const hwt_options_f32 *opts, const hwt
     return HWT_OK;
 }
 
-int64_t hwt_batch_device_memory(const hwt_batch *b) { return b ? (int64_t)b->bytes : 0; }
+int64_t hwt_batch_device_memory(const hwt_batch *b)
+{
+    return b ? (int64_t)(b->bytes + (b->plan ? b->plan->ws_bytes : 0)) : 0;
+}
 
 void hwt_batch_destroy(hwt_batch *b)
 {
@@ -1403,7 +1409,15 @@ int hwt_batch_price(hwt_batch *b, double *host_out, const hwt_config *cfg, hwt_s
     int rc = g.enter(b->device);
     if (rc) return rc;
     if (c.lane_max_width <= 0) c.lane_max_width = std::max(b->max_width, 3);
-    return price_views_once(b->o, b->y, b->out, host_out, c, stats);
+    if (b->once) c.flags |= HWT_FLAG_NO_GRAPH;  // one execution: a graph would not be replayed
+    if (!b->plan || std::memcmp(&b->plan_cfg, &c, sizeof(c)) != 0) {
+        if (b->plan) hwt_plan_destroy(b->plan);
+        b->plan = nullptr;
+        rc = hwt_plan_create(&b->plan, b->o.n, b->y.n, &c);
+        if (rc) return rc;
+        b->plan_cfg = c;
+    }
+    return price_views(b->plan, b->o, b->y, b->out, host_out, stats);
 }
 
 int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, double *out_prices, const hwt_config *cfg,
@@ -1423,6 +1437,7 @@ int hwt_price_f64(const hwt_options_f64 *opts, const hwt_yield_f64 *yield, doubl
     hwt_batch *b = nullptr;
     rc = hwt_batch_create(&b, opts, yield);
     if (rc) return rc;
+    b->once = true;
     rc = hwt_batch_price(b, out_prices, &c, stats);
     hwt_batch_destroy(b);
     return rc;
@@ -1445,6 +1460,7 @@ static int price_constants(const hwt_option_constants_f64 *consts, int32_t n, co
     hwt_batch *b = nullptr;
     rc = new_batch(&b, b);
     if (rc) return rc;
+    b->once = true;
     rc = batch_of_constants(b, consts, n, yield, f32);
     if (rc == HWT_OK) rc = hwt_batch_price(b, host_out, &c, stats);
     hwt_batch_destroy(b);

@@ -122,6 +122,13 @@ def test_batch_uploaded_once_prices_many_times(pkg, oracle):
                                                             (C.FORM_LANE, "-"), (C.FORM_AUTO, "W"))]
     for got in outs:
         assert np.all(np.abs(got - ref) <= parity_tol(ref))
+    # the same settings again: the batch's kept plan is reused (no new device
+    # memory) and replays the same prices bit for bit
+    held = b.device_memory
+    again = [b.price(C.config(form=C.FORM_AUTO, sort="W")) for _ in range(3)]
+    assert b.device_memory == held
+    for got in again:
+        assert np.array_equal(got, outs[-1])
     b.close()
 
 
