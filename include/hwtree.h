@@ -257,7 +257,9 @@ int hwt_price_constants_f32(const hwt_option_constants_f32 *consts, int32_t n, c
  * hwt_batch_price prices the resident batch into host_out (input order), with
  * no re-upload: the batch keeps the plan of its last call and reuses it, and
  * its captured graph, while the settings stay the same (a change of settings
- * builds a new plan).  hwt_batch_device_memory counts the kept plan too. */
+ * builds a new plan).  hwt_batch_device_memory counts the kept plan too.
+ * Calls on one batch must not overlap (from several host threads): they
+ * share its device output array and its plan. */
 typedef struct hwt_batch hwt_batch;
 int hwt_batch_create(hwt_batch **batch, const hwt_options_f64 *host_opts, const hwt_yield_f64 *host_yield);
 /* The same for a real = float caller's arrays: the batch holds each option's
