@@ -382,14 +382,16 @@ __device__ __forceinline__ void init_window(T (&z)[K], const TreeShape &t, int b
 // transformed ones: the set-up needs no more registers than the steps, and
 // the form fits every window size at two waves per SIMD.  Undoing D after the
 // steps needs the scales again: each lane parks them in its own node slots of
-// the LDS state buffer, which the steps leave untouched.  FP64 only.
+// the LDS state buffer, which the steps leave untouched.  In mixed mode the
+// coefficients and scales are formed in FP64 and rounded to FP32, and the
+// state and the steps are FP32.
 
 // Transformed coefficients of this lane's nodes, and z -> D^{-1} z: the chain
 // d_s = c_s d_{s-1} (c_s = cl_s on tree nodes, 1 on padding) is built going
 // up with cr'_{s-1} = cr_{s-1} c_s, then walked down with 1/d_{s-1} = c_s / d_s.
-template <int K, bool FWD>
-__device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, const lds_f64 *etab,
-                                        double (&z)[K], double &cl0, double (&um)[K], double (&ur)[K], lds_f64 *lds)
+template <class T, int K, bool FWD>
+__device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, const lds_f64 *etab, T (&z)[K],
+                                        T &cl0, T (&um)[K], T (&ur)[K], typename Lds<T>::type *lds)
 {
     const lds_f64 *tab = etab + (FWD ? 0 : 18);
     const int k0 = base + lane * K;
@@ -415,7 +417,7 @@ __device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, 
         pm_cur = (2.0 / 3.0 - a2) * Ek;
     }
     double c[K];  // the chain factors, kept for the walk down
-    double cl, cm, cr, cr_prev = 0.0, d = 1.0;
+    double cl, cm, cr, cr_prev = 0.0, d = 1.0, cl0d = 0.0;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
         const int k = k0 + s;
@@ -446,50 +448,52 @@ __device__ __forceinline__ void unit_in(const TreeShape &t, int base, int lane, 
                 cr = tab[e + 2];
             }
         }
-        um[s] = cm;
+        um[s] = (T)cm;
         if (s == 0) {
-            cl0 = cl;
+            cl0d = cl;
         } else {
             c[s] = (k < t.w) ? cl : 1.0;
             d *= c[s];
-            ur[s - 1] = cr_prev * c[s];
-            lds[k] = d;
+            ur[s - 1] = (T)(cr_prev * c[s]);
+            lds[k] = (T)d;
         }
         cr_prev = cr;
         Eprev = Ek;
         Ek = Enext;
     }
     double inv = 1.0 / d;
-    ur[K - 1] = cr_prev * inv;
-    cl0 *= dpp_from_lower_lane(d);
+    ur[K - 1] = (T)(cr_prev * inv);
+    cl0 = (T)(cl0d * dpp_from_lower_lane(d));
 #pragma unroll
     for (int s = K - 1; s > 0; --s) {
-        z[s] *= inv;
+        z[s] = (T)((double)z[s] * inv);
         inv *= c[s];
     }
 }
 
 // z -> D z with the scales unit_in parked in LDS.
-template <int K>
-__device__ __forceinline__ void unit_out(int base, int lane, double (&z)[K], const lds_f64 *lds)
+template <class T, int K>
+__device__ __forceinline__ void unit_out(int base, int lane, T (&z)[K], const typename Lds<T>::type *lds)
 {
     const int k0 = base + lane * K;
 #pragma unroll
     for (int s = 1; s < K; ++s) z[s] *= lds[k0 + s];
 }
 
-// Largest window the unit form is used for.  FP64: every window (21 nodes
-// per lane).  FP32 (mixed mode): not used -- the undiscounted
-// backward state of a long, weakly mean-reverting bond already comes within a
-// few powers of ten of the FP32 limit (about 1e34 for a 100-year bond with
-// a = 0.009), and D^{-1} would scale it past.
+// Largest window the unit form is used for: every window (21 nodes per
+// lane), in both precisions.  In FP32, D^{-1} (scales down to 1e-30, checked
+// per option by the setup kernel) can lift the undiscounted backward state of
+// a long, weakly mean-reverting bond past FP32's range; such an option's sums
+// or exercise-node values leave the range guard's bound, and it is re-priced
+// by the FP64 kernel like any other out-of-range option.
 template <class T> struct UnitMaxK;
 template <> struct UnitMaxK<double> { static constexpr int value = MAXK; };
-template <> struct UnitMaxK<float> { static constexpr int value = 1; };
+template <> struct UnitMaxK<float> { static constexpr int value = MAXK; };
 
-// a*b + c into a fresh register.  Written as the three-address v_fma_f64:
-// left to itself the compiler accumulates into c's register (v_fmac) and then
-// spends a 64-bit move per node restoring the loop-carried registers.  The
+// a*b + c into a fresh register.  Written as the three-address v_fma_f64 /
+// v_fma_f32: left to itself the compiler accumulates into c's register
+// (v_fmac) and then spends a move per node restoring the loop-carried
+// registers (in FP32 as many moves as the unit form saves multiplies).  The
 // result only feeds an ordinary FMA, which is what the DPP shifts read.
 __device__ __forceinline__ double fma3(double a, double b, double c)
 {
@@ -497,7 +501,12 @@ __device__ __forceinline__ double fma3(double a, double b, double c)
     asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
     return d;
 }
-__device__ __forceinline__ float fma3(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ float fma3(float a, float b, float c)
+{
+    float d;
+    asm("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
 
 template <class T, int K>
 __device__ __forceinline__ void unit_step(const T (&src)[K], T (&dst)[K], T cl0, const T (&cm)[K], const T (&cr)[K])
@@ -602,10 +611,10 @@ __device__ __forceinline__ void phase_fn(const TreeShape &t, int base, int nstep
         bool done = false;
         if constexpr (K > 1 && K <= UnitMaxK<T>::value) {
             if (unit) {
-                double cl0, um[K], ur[K];
-                unit_in<K, FWD>(t, base, lane, etab, z, cl0, um, ur, lds);
-                unit_steps<double, K>(z, cl0, um, ur, nsteps);
-                unit_out<K>(base, lane, z, lds);
+                T cl0, um[K], ur[K];
+                unit_in<T, K, FWD>(t, base, lane, etab, z, cl0, um, ur, lds);
+                unit_steps<T, K>(z, cl0, um, ur, nsteps);
+                unit_out<T, K>(base, lane, z, lds);
                 done = true;
             }
         }

@@ -327,8 +327,9 @@ def test_wide_step_counts(pkg, precision):
     slots cover the active tree region (the work estimate) with padding on
     top; the operations follow from the forms: 4K+1 per lane and step in the
     unit-diagonal form (between 4 and 5 per slot), 5 per slot in the plain
-    form.  FP64 steps every window of these trees in the unit form, mixed
-    precision none."""
+    form.  FP64 steps every window of these trees in the unit form; mixed
+    precision steps the same windows, in the unit form wherever the setup
+    kernel's FP32 scale check passes (here: every window), else plain."""
     capi = pkg.capi
     y = pkg.datasets.synthetic_yield()
     opts = pkg.datasets.skewed(seed=3, n=3000)
@@ -340,8 +341,10 @@ def test_wide_step_counts(pkg, precision):
     assert 4 * unit + 5 * plain <= ops <= 5 * (unit + plain)
     if precision == "fp64":
         assert plain == 0 and unit > 0
-    elif st["fp64_repriced"] == 0:
-        assert unit == 0 and ops == 5 * plain
+    else:
+        _, s64 = capi.price(opts, y, capi.config(form=capi.FORM_WAVE), stats=True)
+        assert unit + plain == s64["wide_unit_slots"]  # the same windows and steps
+        assert unit > 0 and plain == 0
 
 
 def test_plan_capacity_enforced(pkg):
