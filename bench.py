@@ -240,6 +240,7 @@ def main():
     ap.add_argument("--concurrent", action="store_true", help="--profile: narrow-tree kernel beside the wide one")
     ap.add_argument("--no-rows", action="store_true", help="--profile: narrow trees one per wavefront")
     ap.add_argument("--fused", action="store_true", help="--profile: one persistent kernel for every tree")
+    ap.add_argument("--mixed", action="store_true", help="--profile: mixed precision (FP32 tree passes)")
     ap.add_argument("--profile-lane", default=None, metavar="SET",
                     help="form 1 alone, without the graph, on one of the reference's 65,536-option sets "
                          "(e.g. 0_UNIFORM), for a profiler")
@@ -298,7 +299,8 @@ def main():
         dm.synchronize()
         flags = capi.FLAG_NO_GRAPH | (capi.FLAG_CONCURRENT if args.concurrent else 0) | \
             (capi.FLAG_NO_ROWS if args.no_rows else 0) | (capi.FLAG_FUSED if args.fused else 0)
-        plan = capi.Plan(opts.N, y.N, capi.config(sort=args.sort, flags=flags))
+        prec = capi.PREC_MIXED if args.mixed else capi.PREC_FP64
+        plan = capi.Plan(opts.N, y.N, capi.config(sort=args.sort, flags=flags, precision=prec))
         wide = []
         for i in range(args.warmup + args.steps):
             plan.execute(dev)
@@ -307,8 +309,9 @@ def main():
                 wide.append(st["wide_ms"])
             print(f"profile step {i}: wide kernel {st['wide_ms']:.4f} ms", flush=True)
         plan.close()
-        print(json.dumps({"profile": "skewed_60k", "copy_bytes": n, "wide_ms_mean": float(np.mean(wide)),
-                          "wide_node_updates": st["wide_node_updates"]}), flush=True)
+        print(json.dumps({"profile": "skewed_60k", "precision": "mixed" if args.mixed else "fp64", "copy_bytes": n,
+                          "wide_ms_mean": float(np.mean(wide)), "wide_node_updates": st["wide_node_updates"],
+                          "fp64_repriced": st["fp64_repriced"]}), flush=True)
         return
 
     # whole-job throughput: executions on a caller stream (graph replays or direct
